@@ -1,0 +1,187 @@
+"""Benchmark: smoke cell-steps/s on one MI355X, 2048x2048 multigrid plume.
+
+Workload (plume_2048_mg): 2048x2048 MAC grid, float64 fields, dx = 1/2048,
+dt = 0.005, buoyancy f_y = 1.0*density, 4 density boxes of 48x48 cells
+(strength U(0.5,1.5), lower third), initial velocity = curl of 32 Gaussian
+blobs (seed 3).  Pressure: geometric multigrid, V(2,2) red-black
+Gauss-Seidel (omega = 1), full-weighting (masked 2x2 mean) restriction,
+bilinear prolongation, 8 levels down to 16x16 solved by 50 RB-GS sweeps,
+V-cycles until ||r|| / ||alpha*div|| <= 1e-8.
+
+A step is one full SmokeSimulation2D.substep (sources, buoyancy, advection of
+u, v and density, divergence, pressure solve, projection + walls, metrics
+divergence and residual) with all fields resident in HBM.  Two variants:
+``fp64`` (FP64 V-cycles) and ``mixed`` (FP32 V-cycles inside FP64 defect
+correction, same FP64 tolerance).  ``value`` is the mixed variant's
+cell-steps/s; the fp64 figures are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "python-game-engine_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_cell(v_cycles_per_step: float, mixed: bool, levels: int,
+                               refinements: float = 0.0) -> float:
+    """Bytes per cell-step by logical pass (float64 8 B, float32 4 B):
+    non-solve passes 168, metrics divergence + residual 48, per FP64 V-cycle
+    176.06 (V(2,2) smoothing 4x24, residual+restrict 18, prolong+correct 18,
+    summed over levels x 4/3, plus 50 coarsest sweeps) + 16 for the norm
+    check; mixed: per FP32 V-cycle 88.03, per refinement 40."""
+    geo = sum(0.25 ** l for l in range(levels - 1))
+    coarse_cells = 0.25 ** (levels - 1)
+    per_vc64 = 132.0 * geo + 50 * 24.0 * coarse_cells
+    if mixed:
+        per_vc = per_vc64 / 2.0
+        return 168.0 + 48.0 + v_cycles_per_step * per_vc + refinements * 40.0 + 16.0
+    return 168.0 + 48.0 + v_cycles_per_step * (per_vc64 + 16.0) + 16.0
+
+
+def run_variant(scene, precision: str, steps: int, warmup: int):
+    import torch
+
+    from cfd_amd import scenes
+
+    sim = scenes.make_simulation(scene, dtype="float64", device="cuda",
+                                 solver_overrides={"precision": precision, "inner_cycles": 1})
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        sim.substep(scene.dt)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    cycles, refinements = [], []
+    start.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.substep(scene.dt)
+        info = sim.last_solve_info
+        cycles.append(info.get("iterations", 0))
+        refinements.append(info.get("refinements", 0))
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(end)
+    return {
+        "ms_per_step": ms / steps,
+        "wall_ms_per_step": wall * 1e3 / steps,
+        "cell_steps_per_s": scene.cells * steps / (ms / 1e3),
+        "v_cycles_per_step": sum(cycles) / steps,
+        "refinements_per_step": sum(refinements) / steps,
+        "rel_residual_last": sim.last_solve_info.get("rel_residual"),
+        "levels": sim.last_solve_info.get("levels"),
+    }, sim
+
+
+def cpu_baseline(scene, sample_steps: int):
+    import oracle
+    import oracle.sim
+
+    ref = oracle.sim.from_scene(scene)
+    ref.substep(scene.dt)  # warm-up (page-in, thread start)
+    t0 = time.perf_counter()
+    cyc = 0
+    for _ in range(sample_steps):
+        ref.substep(scene.dt)
+        cyc += ref.last_solve["iterations"]
+    dt = time.perf_counter() - t0
+    return {"value": scene.cells * sample_steps / dt, "unit": "cell-steps/s",
+            "cores": oracle.num_threads(), "kind": "port",
+            "sample": (f"{sample_steps} steps of the same {scene.H}x{scene.W} workload after 1 "
+                       f"warm-up step, FP64 oracle (C/OpenMP restatement), "
+                       f"{cyc / sample_steps:.1f} V-cycles/step, {dt:.1f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--N", type=int, default=2048)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variants", default="fp64,mixed")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from cfd_amd import scenes
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+
+    log(f"generating plume_{args.N}_mg")
+    scene = scenes.plume_2048_mg(N=args.N)
+    results = {}
+    for var in args.variants.split(","):
+        log(f"variant {var}: warmup {args.warmup}, steps {args.steps}")
+        r, _ = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
+                           args.warmup)
+        r["alg_bytes_per_cell_step"] = algorithmic_bytes_per_cell(
+            r["v_cycles_per_step"], var == "mixed", r["levels"] or 8, r["refinements_per_step"])
+        r["alg_GBps_whole_step"] = r["alg_bytes_per_cell_step"] * r["cell_steps_per_s"] / 1e9
+        results[var] = r
+        log(f"variant {var}: {json.dumps(r)}")
+
+    main_var = "mixed" if "mixed" in results else list(results)[0]
+    m = results[main_var]
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: {args.cpu_steps} oracle steps")
+        cpu = cpu_baseline(scene, args.cpu_steps)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "smoke cell-steps/s & %HBM roofline at 2048² MG (mixed vs FP64); grad/fwd ratio",
+        "value": m["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64/f32-mixed" if main_var == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plume_{args.N}_mg", "grid": [args.N, args.N],
+                   "solver": "multigrid V(2,2) RB-GS, 8 levels, coarsest 50 sweeps, tol 1e-8, "
+                             "rediscretized coarse scale",
+                   "variant": main_var},
+        "variants": results,
+        "roofline": {"bound": "hbm", "achieved": m["alg_GBps_whole_step"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": m["alg_GBps_whole_step"] / PEAK_HBM_GBS,
+                     "traffic": None, "scope": "whole step (provisional)"},
+        "cpu_baseline": cpu,
+    }
+    if "fp64" in results and "mixed" in results:
+        line["mixed_speedup_vs_fp64"] = results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

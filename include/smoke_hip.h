@@ -1,0 +1,227 @@
+/*
+ * smoke_hip.h -- C ABI of libsmoke_hip.so, the gfx950 smoke-step library.
+ *
+ * Each entry point replaces one (operation, backend) implementation that the
+ * reference registers behind its operator / solver registries
+ * (src/cfd/operators/registrations.py:21-29, src/cfd/solvers/registrations.py:17-20);
+ * the comment above each names the reference kernel it replaces
+ * (paths relative to the reference root, LCEBurton/python-game-engine).
+ *
+ * Conventions
+ *  - Pointers are device pointers (HBM) owned by the caller; the library never
+ *    frees caller memory.  Cell fields are H x W, u faces H x (W+1), v faces
+ *    (H+1) x W, row-major.  Masks are uint8 0/1 (torch.bool storage).
+ *  - dtype: SMK_F64 or SMK_F32; every floating-point pointer of one call has
+ *    that type.
+ *  - stream: a hipStream_t (NULL = default stream).  Calls only enqueue work,
+ *    except where a comment says the call reads a result back (solver
+ *    tolerance checks, solve info).
+ *  - Return: 0 on success, < 0 invalid argument (SMK_EINVAL), > 0 a
+ *    hipError_t.  smk_last_error() describes the last failure of the calling
+ *    thread.
+ *  - Kernels are deterministic: the same inputs give bit-identical outputs.
+ */
+#ifndef SMOKE_HIP_H
+#define SMOKE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMK_F64 0
+#define SMK_F32 1
+
+#define SMK_OK 0
+#define SMK_EINVAL (-1)
+
+/* Thread-local description of the last error (never NULL). */
+const char* smk_last_error(void);
+/* ABI version, bumped on incompatible change. */
+int smk_abi_version(void);
+
+/* ---------------------------------------------------------------- masks */
+
+/* simulation.py:106-108 / :200-202 and multigrid/masks.py:30-49:
+ * face_mask_u[:,1:-1] = ~(solid[:,:-1] | solid[:,1:]), outer faces 0;
+ * likewise v.  Writes every element of fmu and fmv. */
+int smk_face_masks_from_solid(const uint8_t* solid, int H, int W, uint8_t* fmu,
+                              uint8_t* fmv, void* stream);
+/* multigrid/masks.py:52-74 (== jacobi.py:46): is_fluid = OR of 4 faces. */
+int smk_is_fluid(const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                 uint8_t* is_fluid, void* stream);
+/* Packed per-cell stencil flags used by the solvers (no reference
+ * counterpart: one byte replaces the three masks read by residual.py:35-38). */
+int smk_cell_flags(const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                   uint8_t* flags, void* stream);
+/* multigrid/masks.py:3-27: coarse solid = any solid in the 2x2 block. */
+int smk_coarsen_solid(const uint8_t* solid_fine, int H, int W,
+                      uint8_t* solid_coarse, void* stream);
+
+/* ------------------------------------------------------------ advection */
+
+/* backends/numba/operators/advection.py:112-128 advect_scalar_field
+ * (operator key "advection_scalar").  u is uh x uw, v is vh x vw (the
+ * samplers clamp to each array's own shape, advection.py:24).  dst =
+ * sample * scale; scale = 1 is the reference operator, scale != 1 fuses the
+ * density dissipation of simulation.py:232. */
+int smk_advect_scalar(const void* src, void* dst, int H, int W, const void* u,
+                      int uh, int uw, const void* v, int vh, int vw,
+                      const uint8_t* is_fluid, double dt, double cell_size,
+                      double scale, int dtype, void* stream);
+/* advection.py:130-154 advect_velocity_u ("advection_velocity_u"). */
+int smk_advect_u(const void* u_src, void* u_dst, int uh, int uw, const void* v,
+                 int vh, int vw, const uint8_t* fmu, double dt,
+                 double cell_size, int dtype, void* stream);
+/* advection.py:156-180 advect_velocity_v ("advection_velocity_v"). */
+int smk_advect_v(const void* v_src, void* v_dst, int vh, int vw, const void* u,
+                 int uh, int uw, const uint8_t* fmv, double dt,
+                 double cell_size, int dtype, void* stream);
+/* Both of the above in one launch on MAC-shaped arrays, as
+ * simulation.py:280-286 calls them (both read the old velocity). */
+int smk_advect_velocity(const void* u_src, void* u_dst, const void* v_src,
+                        void* v_dst, const uint8_t* fmu, const uint8_t* fmv,
+                        int H, int W, double dt, double cell_size, int dtype,
+                        void* stream);
+
+/* --------------------------------------------------------------- forces */
+
+/* forces.py:78-105 apply_forces ("apply_forces"), including the float32
+ * force scratch of forces.py:33,62 and the float32 sum of forces.py:105.
+ * v is vh x vw, density H x W, temperature Ht x Wt or NULL (all zero). */
+int smk_apply_forces(void* v, int vh, int vw, const void* density, int H, int W,
+                     const void* temperature, int Ht, int Wt, double dt,
+                     double gravity, double buoyancy_coefficient,
+                     double density_coefficient, double ambient_temperature,
+                     int dtype, void* stream);
+/* forces.py:108-123 cool_temperature ("cool_temperature"); decay is
+ * exp(-cooling_rate*dt), evaluated once by the caller as forces.py:120 does. */
+int smk_cool_temperature(void* T, int H, int W, double ambient, double decay,
+                         int dtype, void* stream);
+/* simulation.py:232 density *= dissipation (x[k] *= factor, n elements). */
+int smk_scale(void* x, long n, double factor, int dtype, void* stream);
+
+/* ----------------------------------------------------------- projection */
+
+/* divergence.py:4-25 compute_divergence ("compute_divergence"). */
+int smk_divergence(const void* u, const void* v, void* div, const uint8_t* fmu,
+                   const uint8_t* fmv, int H, int W, double cell_size,
+                   int dtype, void* stream);
+/* pressure_gradient.py:4-37 apply_pressure_gradient
+ * ("apply_pressure_gradient").  If solid != NULL the boundary enforcement of
+ * boundaries.py:7-39 is applied to the same faces in the same launch
+ * (simulation.py:244-249 always runs it right after). */
+int smk_pressure_gradient(void* u, void* v, const void* p, const uint8_t* fmu,
+                          const uint8_t* fmv, const uint8_t* solid, int H,
+                          int W, double fluid_density, double cell_size,
+                          double dt, int dtype, void* stream);
+/* boundaries.py:7-39 enforce_boundary_conditions
+ * ("enforce_boundary_conditions"). */
+int smk_enforce_boundaries(void* u, void* v, const uint8_t* solid, int H, int W,
+                           int dtype, void* stream);
+/* residual.py:6-42 compute_residual ("compute_residual"). */
+int smk_residual(const void* p, const void* div, const uint8_t* is_fluid,
+                 const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                 double alpha, double reciprocal_beta, void* r, int dtype,
+                 void* stream);
+
+/* ------------------------------------------------------------- emitters */
+
+/* emitters.py:157-178 (Emitter2D._apply_scalar_field) with the falloffs of
+ * emitters.py:339-411; approach 0 hard_cutoff, 1 linear, 2 quadratic,
+ * 3 gaussian.  Only the bounding box of the disc is visited (cells outside
+ * it get falloff 0, as in emitters.py:52-56). */
+int smk_emit_scalar(void* field, int H, int W, double value, double dt,
+                    double pos_x, double pos_y, double radius, int approach,
+                    int dtype, void* stream);
+/* Box source (no reference counterpart; the plume workloads need it):
+ * field[y0:y1, x0:x1] = value (mode 0) or += value (mode 1). */
+int smk_box_source(void* field, int H, int W, int x0, int y0, int x1, int y1,
+                   double value, int mode, int dtype, void* stream);
+
+/* -------------------------------------------------------------- solvers */
+
+typedef struct smk_solve_info {
+  int iterations;      /* sweeps (Jacobi / RB-SOR) or V-cycles executed   */
+  int converged;       /* 1 if the tolerance was met (tolerance runs)     */
+  double rel_residual; /* ||r|| / ||alpha*div|| over fluid cells, or -1   */
+  int inner_cycles;    /* mixed precision: FP32 V-cycles in total         */
+  int refinements;     /* mixed precision: FP64 defect-correction steps   */
+} smk_solve_info;
+
+/* Device workspace (bytes) the single-grid solvers need for an H x W grid. */
+size_t smk_solver_workspace_bytes(int H, int W, int dtype);
+
+/* jacobi.py:27-52 jacobi_pressure_solver (solver key "jacobi"):
+ * `iterations` sweeps, non-fluid cells copied, result in p. */
+int smk_jacobi_solve(void* p, const void* div, const uint8_t* fmu,
+                     const uint8_t* fmv, int H, int W, double fluid_density,
+                     double cell_size, double dt, int iterations, void* ws,
+                     size_t ws_bytes, int dtype, void* stream);
+
+/* red_black_sor_gauss_seidel.py:30-42 (solver key "rb_sor_gauss_seidel"):
+ * omega <= 0 or > 2 picks calculateOptimalOmega(max(H,W)) (common.py:30-39).
+ * tol <= 0: exactly `iterations` sweeps (the reference).  tol > 0: sweeps in
+ * blocks of check_every until ||r||/||alpha*div|| <= tol or `iterations`
+ * sweeps (new; reads the norm back once per block).  info may be NULL. */
+int smk_rbsor_solve(void* p, const void* div, const uint8_t* fmu,
+                    const uint8_t* fmv, int H, int W, double fluid_density,
+                    double cell_size, double dt, int iterations, double omega,
+                    double tol, int check_every, void* ws, size_t ws_bytes,
+                    smk_solve_info* info, int dtype, void* stream);
+
+/* ||r||^2 over fluid cells of the residual of residual.py:6-42 with
+ * reciprocal_beta = 1/4, reduced on the device in a fixed order and read
+ * back into *norm2 (synchronises the stream). */
+int smk_residual_norm2(const void* p, const void* div, const uint8_t* fmu,
+                       const uint8_t* fmv, int H, int W, double alpha, void* ws,
+                       size_t ws_bytes, double* norm2, int dtype, void* stream);
+
+/* ------------------------------------------------------------ multigrid */
+
+typedef struct smk_mg smk_mg;
+
+#define SMK_SMOOTH_JACOBI 0
+#define SMK_SMOOTH_RBSOR 2
+
+#define SMK_COARSE_REFERENCE 0    /* restrict(r)/alpha(2h), solver.py:124-125 */
+#define SMK_COARSE_REDISCRETIZED 1 /* restrict(r)/alpha(h) */
+
+#define SMK_PREC_NATIVE 0 /* V-cycles in the handle's dtype               */
+#define SMK_PREC_MIXED 1  /* FP32 V-cycles inside FP64 defect correction  */
+
+typedef struct smk_mg_params {
+  int v_cycles;            /* fixed cycles (tolerance <= 0) or the cap    */
+  int smoother;            /* SMK_SMOOTH_*  (MultigridParams.smoother_type) */
+  int smoother_iterations; /* sweeps per pre/post smooth                  */
+  int coarsest_iterations; /* sweeps on the coarsest level                */
+  double smoother_omega;   /* <= 0: the RB-SOR smoother's automatic omega
+                              per level (reference); > 0 forces it        */
+  int coarse_scale;        /* SMK_COARSE_*                                 */
+  double tolerance;        /* > 0: stop at ||r||/||alpha*div|| <= tolerance */
+  int precision;           /* SMK_PREC_*                                   */
+  int inner_cycles;        /* mixed: FP32 V-cycles per refinement          */
+} smk_mg_params;
+
+/* multigrid/solver.py:34-92 (_build_levels, _build_buffers, _ensure_cache):
+ * builds the level hierarchy for these face masks (halving while both sizes
+ * are even, up to num_levels) and allocates every level buffer. */
+int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                  int num_levels, int dtype, void* stream, smk_mg** out);
+/* Number of levels actually built. */
+int smk_mg_num_levels(const smk_mg* mg);
+/* multigrid/solver.py:141-176 MultigridSolver.__call__ (solver key
+ * "multigrid"): p is read as the initial guess and overwritten with the
+ * result; div is the right-hand side.  Tolerance and mixed-precision runs
+ * read the residual norm back once per cycle. */
+int smk_mg_solve(smk_mg* mg, void* p, const void* div, double fluid_density,
+                 double cell_size, double dt, const smk_mg_params* params,
+                 smk_solve_info* info, void* stream);
+int smk_mg_destroy(smk_mg* mg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

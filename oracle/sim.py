@@ -1,0 +1,150 @@
+"""Oracle rollout: the step order of SmokeSimulation2D.substep
+(src/cfd/common/simulation.py:186-265) over the C restatement, float64 numpy.
+
+TEST INFRASTRUCTURE ONLY (checker for cfd_amd.common.simulation).
+
+Reference behaviour kept: masks built before the boundary ring is marked
+solid (simulation.py:102-113), p zeroed before every solve (:310), the solve
+uses self.dt and the gradient the dt argument (:311-317), the metrics stage
+recomputes divergence and residual (:251-265).  Box sources (density :=
+value) are applied at the emitter stage.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+
+
+class OracleSmokeSim:
+    def __init__(self, H, W, cell_size, solver="jacobi", solver_params=None):
+        self.height, self.width = H, W
+        self.cell_size = cell_size
+        self.dt = 0.016
+        self.density_dissipation = 0.999
+        self.buoyancy = 5.0
+        self.gravity = -0.5
+        self.fluid_density = 1.0
+        self.ambient_temperature = 0.0
+        self.cooling_rate = 0.5
+        z = lambda s: np.zeros(s, dtype=np.float64)  # noqa: E731
+        self.density = z((H, W))
+        self.density_temp = z((H, W))
+        self.velocity_u = z((H, W + 1))
+        self.velocity_v = z((H + 1, W))
+        self.velocity_u_temp = z((H, W + 1))
+        self.velocity_v_temp = z((H + 1, W))
+        self.pressure = z((H, W))
+        self.temperature = z((H, W))
+        self.temperature_temp = z((H, W))
+        self.divergence = z((H, W))
+        self.pressure_residual = z((H, W))
+        self.solid = np.zeros((H, W), dtype=bool)
+        self.fmu = np.zeros((H, W + 1), dtype=bool)
+        self.fmv = np.zeros((H + 1, W), dtype=bool)
+        self.fmu[:, 1:-1] = ~(self.solid[:, :-1] | self.solid[:, 1:])
+        self.fmv[1:-1, :] = ~(self.solid[:-1, :] | self.solid[1:, :])
+        self.is_fluid = self.fmu[:, :-1] | self.fmu[:, 1:] | self.fmv[:-1, :] | self.fmv[1:, :]
+        self.solid[0, :] = True
+        self.solid[-1, :] = True
+        self.solid[:, 0] = True
+        self.solid[:, -1] = True
+        self.mask_dirty = False
+        self.solver = solver
+        self.solver_params = dict(solver_params or {})
+        self.boxes = []      # (x0, y0, x1, y1, value)
+        self.emitters = []   # dicts: position, radius, density/temperature/velocity values, approach
+        self.last_solve = None
+
+    def set_solid_mask(self, solid):
+        self.solid = np.asarray(solid, dtype=bool).copy()
+        self.mask_dirty = True
+
+    # simulation.py:186-265
+    def substep(self, dt):
+        if self.mask_dirty:
+            self.fmu[:, 1:-1] = ~(self.solid[:, :-1] | self.solid[:, 1:])
+            self.fmv[1:-1, :] = ~(self.solid[:-1, :] | self.solid[1:, :])
+            self.is_fluid = self.fmu[:, :-1] | self.fmu[:, 1:] | self.fmv[:-1, :] | self.fmv[1:, :]
+            self.mask_dirty = False
+        h = self.cell_size
+        for (x0, y0, x1, y1, val) in self.boxes:
+            self.density[y0:y1, x0:x1] = val
+        for e in self.emitters:
+            px, py = e["position"]
+            for fld, key in ((self.density, "density_value"), (self.temperature, "temperature_value")):
+                if e.get(key, 0.0) != 0.0:
+                    O.emit_scalar(fld, e[key], dt, px, py, e["radius"], e["approach"])
+            vel = e.get("velocity_value", (0.0, 0.0))
+            if vel[0] != 0.0:
+                O.emit_scalar(self.velocity_u, vel[0], dt, px, py, e["radius"], e["approach"])
+            if vel[1] != 0.0:
+                O.emit_scalar(self.velocity_v, vel[1], dt, px, py, e["radius"], e["approach"])
+        O.apply_forces(self.velocity_v, self.density, self.temperature, dt, self.gravity,
+                       self.buoyancy, 1.0, self.ambient_temperature)
+        O.advect_velocity_u(self.velocity_u, self.velocity_u_temp, self.velocity_v, self.fmu, dt, h)
+        O.advect_velocity_v(self.velocity_v, self.velocity_v_temp, self.velocity_u, self.fmv, dt, h)
+        self.velocity_u, self.velocity_u_temp = self.velocity_u_temp, self.velocity_u
+        self.velocity_v, self.velocity_v_temp = self.velocity_v_temp, self.velocity_v
+        O.advect_scalar_field(self.density, self.density_temp, self.velocity_u, self.velocity_v,
+                              self.is_fluid, dt, h)
+        self.density, self.density_temp = self.density_temp, self.density
+        O.advect_scalar_field(self.temperature, self.temperature_temp, self.velocity_u,
+                              self.velocity_v, self.is_fluid, dt, h)
+        self.temperature, self.temperature_temp = self.temperature_temp, self.temperature
+        O.cool_temperature(self.temperature, self.ambient_temperature, self.cooling_rate, dt)
+        self.density *= self.density_dissipation
+        O.compute_divergence(self.velocity_u, self.velocity_v, self.divergence, self.fmu, self.fmv, h)
+        self.pressure.fill(0.0)
+        self._solve()
+        O.apply_pressure_gradient(self.velocity_u, self.velocity_v, self.pressure, self.fmu,
+                                  self.fmv, self.fluid_density, h, dt)
+        O.enforce_boundary_conditions(self.velocity_u, self.velocity_v, self.solid)
+        O.compute_divergence(self.velocity_u, self.velocity_v, self.divergence, self.fmu, self.fmv, h)
+        alpha = O.calculate_alpha(h, self.fluid_density, dt)
+        O.compute_residual(self.pressure, self.divergence, self.is_fluid, self.fmu, self.fmv,
+                           alpha, 0.25, self.pressure_residual)
+
+    def _solve(self):
+        p, sp, h = self.pressure, self.solver_params, self.cell_size
+        args = (p, self.divergence, self.fmu, self.fmv, self.fluid_density, h, self.dt)
+        if self.solver == "jacobi":
+            O.jacobi_pressure_solver(*args, iterations=sp.get("iterations", 40))
+            self.last_solve = {"iterations": sp.get("iterations", 40)}
+        elif self.solver == "gauss_seidel":
+            O.gauss_seidel_pressure_solver(*args, iterations=sp.get("iterations", 40))
+        elif self.solver == "rb_sor_gauss_seidel":
+            tol = sp.get("tolerance", 0.0)
+            if tol > 0:
+                n, rel = O.rbsor_solve_tol(*args, omega=sp.get("omega", 0.0), tol=tol,
+                                           check_every=sp.get("check_every", 20),
+                                           max_sweeps=sp.get("iterations", 40))
+                self.last_solve = {"iterations": n, "rel_residual": rel}
+            else:
+                O.red_black_sor_gauss_seidel_pressure_solver(
+                    *args, iterations=sp.get("iterations", 40), omega=sp.get("omega", 0.0))
+        elif self.solver == "multigrid":
+            n, rel = O.multigrid_solver(
+                *args, v_cycles=sp.get("v_cycles", 2), smoother_type=sp.get("smoother_type", "jacobi"),
+                num_levels=sp.get("num_levels", 4), smoother_iterations=sp.get("smoother_iterations", 2),
+                coarsest_iterations=sp.get("coarsest_iterations", 30),
+                smoother_omega=sp.get("smoother_omega", 0.0),
+                coarse_scale=sp.get("coarse_scale", "reference"), tolerance=sp.get("tolerance", 0.0))
+            self.last_solve = {"iterations": n, "rel_residual": rel}
+        else:
+            raise ValueError(self.solver)
+
+
+def from_scene(scene, solver_overrides=None) -> OracleSmokeSim:
+    """Oracle sim with a scene's grid, constants, initial velocity and boxes."""
+    params = dict(scene.solver_params)
+    if solver_overrides:
+        params.update(solver_overrides)
+    s = OracleSmokeSim(scene.H, scene.W, scene.cell_size, scene.solver, params)
+    s.dt = scene.dt
+    s.gravity = scene.gravity
+    s.density_dissipation = scene.dissipation
+    s.velocity_u[...] = scene.u0
+    s.velocity_v[...] = scene.v0
+    s.boxes = [(b.x0, b.y0, b.x1, b.y1, b.value) for b in scene.boxes]
+    return s

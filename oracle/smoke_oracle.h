@@ -1,0 +1,160 @@
+/*
+ * smoke_oracle.h -- CPU restatement of the reference smoke step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (python-game-engine_amd/) never links or calls it.
+ *
+ * Everything is float64, row-major, C-contiguous; index [j*W + i] is row j
+ * (y), column i (x).  Masks are uint8 0/1.  Each function cites the
+ * reference file:line it restates (paths relative to the reference root).
+ *
+ * Parity pins (see tests/test_oracle_*.py): every function below is checked
+ * against the reference's own known-answer tests restated, and against
+ * closed forms / dense linear algebra that a transcription mistake would
+ * violate.  Functions without such a check are marked "parity unpinned".
+ */
+#ifndef SMOKE_ORACLE_H
+#define SMOKE_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/cfd/backends/numba/operators/advection.py:21-49 (+ :97-110 offsets) */
+double orc_sample(const double* f, int fh, int fw, double x, double y);
+
+/* advection.py:112-128 */
+void orc_advect_scalar(const double* src, double* dst, int H, int W,
+                       const double* u, int uh, int uw,
+                       const double* v, int vh, int vw,
+                       const uint8_t* is_fluid, double dt, double cell_size);
+/* advection.py:130-154  (u_src/u_dst are uh x uw, v is vh x vw) */
+void orc_advect_u(const double* u_src, double* u_dst, int uh, int uw,
+                  const double* v, int vh, int vw,
+                  const uint8_t* fmu, double dt, double cell_size);
+/* advection.py:156-180 */
+void orc_advect_v(const double* v_src, double* v_dst, int vh, int vw,
+                  const double* u, int uh, int uw,
+                  const uint8_t* fmv, double dt, double cell_size);
+
+/* forces.py:19-105.  v is vh x vw, density H x W, temperature Ht x Wt.
+ * temperature may be NULL (treated as an all-zero H x W field). */
+void orc_apply_forces(double* v, int vh, int vw,
+                      const double* density, int H, int W,
+                      const double* temperature, int Ht, int Wt,
+                      double dt, double gravity, double buoyancy_coefficient,
+                      double density_coefficient, double ambient_temperature);
+/* forces.py:108-123 */
+void orc_cool_temperature(double* T, int H, int W, double ambient,
+                          double cooling_rate, double dt);
+
+/* divergence.py:4-25 */
+void orc_divergence(const double* u, const double* v, double* div,
+                    const uint8_t* fmu, const uint8_t* fmv,
+                    int H, int W, double cell_size);
+/* pressure_gradient.py:4-37 */
+void orc_pressure_gradient(double* u, double* v, const double* p,
+                           const uint8_t* fmu, const uint8_t* fmv,
+                           int H, int W, double rho, double cell_size,
+                           double dt);
+/* boundaries.py:7-39 */
+void orc_enforce_boundaries(double* u, double* v, const uint8_t* solid,
+                            int H, int W);
+/* residual.py:6-42 */
+void orc_residual(const double* p, const double* div, const uint8_t* is_fluid,
+                  const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                  double alpha, double reciprocal_beta, double* r);
+
+/* solvers/common.py:6-39 */
+double orc_alpha(double cell_size, double density, double dt);
+double orc_optimal_omega(int size);
+
+/* masks.py:52-74 and the identical expression at jacobi.py:46 */
+void orc_is_fluid(const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                  uint8_t* is_fluid);
+
+/* jacobi.py:8-24 (one sweep src -> dst) */
+void orc_jacobi_sweep(const double* p, double* p_out, const double* div,
+                      const uint8_t* is_fluid, const uint8_t* fmu,
+                      const uint8_t* fmv, int H, int W, double alpha,
+                      double reciprocal_beta);
+/* jacobi.py:27-52 */
+void orc_jacobi_solve(double* p, const double* div, const uint8_t* fmu,
+                      const uint8_t* fmv, int H, int W, double rho,
+                      double cell_size, double dt, int iterations);
+/* red_black_sor_gauss_seidel.py:8-27 (iterations full sweeps, in place) */
+void orc_rbsor_sweeps(double* p, const double* div, const uint8_t* is_fluid,
+                      const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                      double alpha, double reciprocal_beta, double omega,
+                      int iterations);
+/* red_black_sor_gauss_seidel.py:30-42 (omega <= 0 or > 2 -> automatic) */
+void orc_rbsor_solve(double* p, const double* div, const uint8_t* fmu,
+                     const uint8_t* fmv, int H, int W, double rho,
+                     double cell_size, double dt, int iterations,
+                     double omega);
+/* gauss_seidel.py:8-48 (serial lexicographic) */
+void orc_gs_solve(double* p, const double* div, const uint8_t* fmu,
+                  const uint8_t* fmv, int H, int W, double rho,
+                  double cell_size, double dt, int iterations);
+
+/* Squared L2 norm of r over fluid cells (new: the reference has no solver
+ * norm; used by the tolerance-driven loops below). */
+double orc_norm2_fluid(const double* r, const uint8_t* is_fluid, int H, int W);
+
+/* Tolerance-driven RB-SOR: sweeps in blocks of check_every until
+ * ||r|| <= tol * ||alpha*div|| (fluid cells) or max_sweeps.  Returns the
+ * number of sweeps executed; *rel_out receives the final relative residual.
+ * New relative to the reference, which runs a fixed sweep count. */
+int orc_rbsor_solve_tol(double* p, const double* div, const uint8_t* fmu,
+                        const uint8_t* fmv, int H, int W, double rho,
+                        double cell_size, double dt, double omega, double tol,
+                        int check_every, int max_sweeps, double* rel_out);
+
+/* multigrid/masks.py:3-27 */
+void orc_coarsen_solid(const uint8_t* solid_fine, int H, int W,
+                       uint8_t* solid_coarse);
+/* multigrid/masks.py:30-49 (outputs are fully written, outer faces 0) */
+void orc_build_face_masks(const uint8_t* solid, int H, int W, uint8_t* fmu,
+                          uint8_t* fmv);
+/* multigrid/transfer.py:5-36 */
+void orc_restrict(const double* fine, const uint8_t* is_fluid_fine, int H,
+                  int W, double* coarse);
+/* multigrid/transfer.py:38-72 */
+void orc_prolong(const double* coarse, int Hc, int Wc,
+                 const uint8_t* is_fluid_fine, int H, int W, double* fine);
+
+/* smoother ids for the multigrid restatement */
+enum { ORC_SMOOTH_JACOBI = 0, ORC_SMOOTH_GS = 1, ORC_SMOOTH_RBSOR = 2 };
+
+/* multigrid/solver.py:16-177.  One call = reference __call__ with a fresh
+ * cache.  smoother_omega <= 0 reproduces the reference (the smoother picks
+ * its automatic omega per level, solver.py:107); > 0 forces that omega for
+ * the RB-SOR smoother (the BASELINE "RB-GS" setting uses 1.0).
+ * rediscretize = 0 reproduces the reference coarse right-hand side
+ * restrict(r)/alpha(2h) (solver.py:124-125); 1 uses restrict(r)/alpha(h),
+ * the rediscretised coarse operator (new).
+ * tol > 0 switches to a tolerance loop: V-cycles until ||r||/||alpha*div||
+ * <= tol or max v_cycles (new).  Returns the number of V-cycles run. */
+int orc_mg_solve(double* p, const double* div, const uint8_t* fmu,
+                 const uint8_t* fmv, int H, int W, double rho,
+                 double cell_size, double dt, int v_cycles, int smoother,
+                 int num_levels, int smoother_iterations,
+                 int coarsest_iterations, double smoother_omega,
+                 int rediscretize, double tol, double* rel_out);
+
+/* emitters.py:6-35 falloff, approach 0 hard_cutoff, 1 linear, 2 quadratic,
+ * 3 gaussian */
+double orc_falloff(double distance, double radius, int approach);
+/* emitters.py:157-178 (default python path: every cell) */
+void orc_emit_scalar(double* field, int H, int W, double value, double dt,
+                     double pos_x, double pos_y, double radius, int approach);
+
+/* number of OpenMP threads the oracle uses */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

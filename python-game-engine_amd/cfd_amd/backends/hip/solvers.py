@@ -1,0 +1,183 @@
+"""HIP pressure solvers (backend key ``"hip"``).
+
+Signatures follow the reference's PressureSolver Protocol
+(``src/cfd/solvers/base.py:5-19``): ``fn(pressure, divergence, face_mask_u,
+face_mask_v, fluid_density, cell_size, dt, **solver_kwargs)`` updating
+``pressure`` in place.  Extra keyword arguments the reference does not have
+(``tolerance``, ``check_every``, ``smoother_omega``, ``coarse_scale``,
+``precision``, ``inner_cycles``) default to the reference behaviour.
+
+The statistics of the last solve (sweeps or V-cycles, relative residual) are
+kept in :data:`LAST_SOLVE_INFO`.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._lib import (SMK_COARSE_REDISCRETIZED, SMK_COARSE_REFERENCE, SMK_PREC_MIXED,
+                   SMK_PREC_NATIVE, SMK_SMOOTH_JACOBI, SMK_SMOOTH_RBSOR, MgParams, SolveInfo,
+                   call, lib, stream_of)
+from .operators import _field, _mask, _shape, dtype_code
+
+LAST_SOLVE_INFO: dict = {}
+
+_WS: dict = {}
+
+
+def _workspace(device: torch.device, nbytes: int) -> torch.Tensor:
+    """Grow-only device scratch shared by the single-grid solvers of one
+    device; stream-ordered use on the current stream makes reuse safe."""
+    key = (device.type, device.index)
+    ws = _WS.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+        _WS[key] = ws
+    return ws
+
+
+def _check(pressure, divergence, face_mask_u, face_mask_v):
+    _field(pressure, "pressure")
+    H, W = pressure.shape
+    _field(divergence, "divergence", pressure.dtype)
+    _shape(divergence, (H, W), "divergence")
+    _mask(face_mask_u, "face_mask_u", (H, W + 1))
+    _mask(face_mask_v, "face_mask_v", (H + 1, W))
+    return H, W
+
+
+def jacobi_pressure_solver(pressure, divergence, face_mask_u, face_mask_v, fluid_density,
+                           cell_size, dt, iterations, **_ignored):
+    """jacobi.py:27-52 (iterations sweeps, LDS-tiled, several per launch)."""
+    H, W = _check(pressure, divergence, face_mask_u, face_mask_v)
+    code = dtype_code(pressure)
+    nb = lib().smk_solver_workspace_bytes(H, W, code)
+    ws = _workspace(pressure.device, nb)
+    call("smk_jacobi_solve", pressure.data_ptr(), divergence.data_ptr(), face_mask_u.data_ptr(),
+         face_mask_v.data_ptr(), H, W, float(fluid_density), float(cell_size), float(dt),
+         int(iterations), ws.data_ptr(), ws.numel(), code, stream_of(pressure))
+    LAST_SOLVE_INFO.clear()
+    LAST_SOLVE_INFO.update(method="jacobi", iterations=int(iterations))
+
+
+def red_black_sor_gauss_seidel_pressure_solver(pressure, divergence, face_mask_u, face_mask_v,
+                                               fluid_density, cell_size, dt, iterations,
+                                               omega=0.0, tolerance=0.0, check_every=20,
+                                               **_ignored):
+    """red_black_sor_gauss_seidel.py:30-42.
+
+    omega <= 0 or > 2 selects calculateOptimalOmega(max(H, W)).  With
+    tolerance > 0 the sweeps stop once ||r||/||alpha*div|| <= tolerance
+    (checked every ``check_every`` sweeps; ``iterations`` is then the cap)."""
+    H, W = _check(pressure, divergence, face_mask_u, face_mask_v)
+    code = dtype_code(pressure)
+    nb = lib().smk_solver_workspace_bytes(H, W, code)
+    ws = _workspace(pressure.device, nb)
+    info = SolveInfo()
+    call("smk_rbsor_solve", pressure.data_ptr(), divergence.data_ptr(), face_mask_u.data_ptr(),
+         face_mask_v.data_ptr(), H, W, float(fluid_density), float(cell_size), float(dt),
+         int(iterations), float(omega), float(tolerance), int(check_every), ws.data_ptr(),
+         ws.numel(), ctypes.byref(info), code, stream_of(pressure))
+    LAST_SOLVE_INFO.clear()
+    LAST_SOLVE_INFO.update(method="rb_sor_gauss_seidel", **info.as_dict())
+
+
+def residual_norm2(pressure, divergence, face_mask_u, face_mask_v, alpha) -> float:
+    """||r||^2 over fluid cells (r of residual.py:6-42 with 1/beta = 1/4),
+    reduced on the device; synchronises."""
+    H, W = _check(pressure, divergence, face_mask_u, face_mask_v)
+    code = dtype_code(pressure)
+    ws = _workspace(pressure.device, lib().smk_solver_workspace_bytes(H, W, code))
+    out = ctypes.c_double(0.0)
+    call("smk_residual_norm2", pressure.data_ptr(), divergence.data_ptr(), face_mask_u.data_ptr(),
+         face_mask_v.data_ptr(), H, W, float(alpha), ws.data_ptr(), ws.numel(), ctypes.byref(out),
+         code, stream_of(pressure))
+    return out.value
+
+
+_SMOOTHERS = {"jacobi": SMK_SMOOTH_JACOBI, "rb_sor_gauss_seidel": SMK_SMOOTH_RBSOR}
+_COARSE = {"reference": SMK_COARSE_REFERENCE, "rediscretized": SMK_COARSE_REDISCRETIZED}
+_PREC = {"native": SMK_PREC_NATIVE, "mixed": SMK_PREC_MIXED}
+
+
+class _MgHandle:
+    def __init__(self, face_mask_u, face_mask_v, num_levels, dtype_code_):
+        H, W = face_mask_u.shape[0], face_mask_u.shape[1] - 1
+        h = ctypes.c_void_p()
+        call("smk_mg_create", face_mask_u.data_ptr(), face_mask_v.data_ptr(), H, W,
+             int(num_levels), int(dtype_code_), stream_of(face_mask_u), ctypes.byref(h))
+        self.ptr = h
+        self.levels = lib().smk_mg_num_levels(h)
+
+    def close(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().smk_mg_destroy(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MultigridSolver:
+    """multigrid/solver.py:16-177 on the GPU.
+
+    Level masks and buffers are cached per instance and rebuilt only when the
+    grid shape, level count, dtype or device changes or ``rebuild_masks`` is
+    passed -- the reference's cache rule (solver.py:87-92), which also means a
+    change of mask *content* under the same shape needs ``rebuild_masks=True``.
+
+    Smoothing is fused into the V-cycle kernels, so ``smoother_backend`` is
+    accepted for signature compatibility and the smoother always runs on the
+    GPU.  ``smoother_type`` is "jacobi" or "rb_sor_gauss_seidel" (the serial
+    lexicographic Gauss-Seidel smoother has no GPU form).
+    """
+
+    def __init__(self):
+        self._key = None
+        self._handle = None
+
+    @property
+    def num_levels_built(self) -> int:
+        return self._handle.levels if self._handle is not None else 0
+
+    def _ensure(self, face_mask_u, face_mask_v, num_levels, pressure, rebuild):
+        key = (tuple(face_mask_u.shape), int(num_levels), pressure.dtype, pressure.device)
+        if rebuild or self._handle is None or self._key != key:
+            if self._handle is not None:
+                torch.cuda.current_stream(pressure.device).synchronize()
+                self._handle.close()
+            self._handle = _MgHandle(face_mask_u, face_mask_v, num_levels, dtype_code(pressure))
+            self._key = key
+
+    def __call__(self, pressure, divergence, face_mask_u, face_mask_v, fluid_density, cell_size,
+                 dt, v_cycles=2, smoother_type="jacobi", smoother_backend="hip", num_levels=4,
+                 smoother_iterations=2, coarsest_iterations=50, rebuild_masks=False,
+                 profiler=None, smoother_omega=0.0, coarse_scale="reference", tolerance=0.0,
+                 precision="native", inner_cycles=1, **solver_kwargs):
+        _check(pressure, divergence, face_mask_u, face_mask_v)
+        if smoother_type not in _SMOOTHERS:
+            raise ValueError(
+                f"multigrid smoother {smoother_type!r} has no GPU form; use one of {list(_SMOOTHERS)}")
+        if coarse_scale not in _COARSE:
+            raise ValueError(f"coarse_scale must be one of {list(_COARSE)}")
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {list(_PREC)}")
+        if precision == "mixed" and pressure.dtype != torch.float64:
+            raise ValueError("mixed precision needs float64 pressure/divergence")
+        self._ensure(face_mask_u, face_mask_v, num_levels, pressure, rebuild_masks)
+        prm = MgParams(v_cycles=int(v_cycles), smoother=_SMOOTHERS[smoother_type],
+                       smoother_iterations=int(smoother_iterations),
+                       coarsest_iterations=int(coarsest_iterations),
+                       smoother_omega=float(smoother_omega), coarse_scale=_COARSE[coarse_scale],
+                       tolerance=float(tolerance), precision=_PREC[precision],
+                       inner_cycles=int(inner_cycles))
+        info = SolveInfo()
+        call("smk_mg_solve", self._handle.ptr, pressure.data_ptr(), divergence.data_ptr(),
+             float(fluid_density), float(cell_size), float(dt), ctypes.byref(prm),
+             ctypes.byref(info), stream_of(pressure))
+        LAST_SOLVE_INFO.clear()
+        LAST_SOLVE_INFO.update(method="multigrid", levels=self._handle.levels, **info.as_dict())

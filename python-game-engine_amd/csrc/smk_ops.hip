@@ -1,0 +1,645 @@
+// smk_ops.hip -- pointwise / stencil / gather operators of the smoke step.
+//
+// One thread per output element on 64x4 blocks: a wavefront covers 64
+// consecutive columns of one row, so every field read and write is a
+// coalesced row segment.  These kernels are HBM-bound (0.1-0.5 flop/byte);
+// no LDS staging is needed because each input element is reused only by
+// its 4-neighbours, which the L2 already serves.
+#include <cmath>
+#include <cstdarg>
+#include <mutex>
+
+#include "smk_common.hpp"
+
+namespace smk {
+
+static thread_local std::string g_last_error = "no error";
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return fail((int)e, "launch of %s failed: %s", what, hipGetErrorString(e));
+  return SMK_OK;
+}
+
+constexpr int BX = 64, BY = 4;
+
+inline dim3 grid2d(int rows, int cols) { return dim3(cdiv(cols, BX), cdiv(rows, BY)); }
+
+// --------------------------------------------------------------- masks
+__global__ void k_face_masks(const uint8_t* __restrict__ solid, int H, int W,
+                             uint8_t* __restrict__ fmu, uint8_t* __restrict__ fmv) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j < H && i <= W) {
+    uint8_t m = 0;
+    if (i >= 1 && i <= W - 1) m = !(solid[(size_t)j * W + i - 1] || solid[(size_t)j * W + i]);
+    fmu[(size_t)j * (W + 1) + i] = m;
+  }
+  if (j <= H && i < W) {
+    uint8_t m = 0;
+    if (j >= 1 && j <= H - 1) m = !(solid[(size_t)(j - 1) * W + i] || solid[(size_t)j * W + i]);
+    fmv[(size_t)j * W + i] = m;
+  }
+}
+
+__global__ void k_is_fluid(const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv,
+                           int H, int W, uint8_t* __restrict__ out) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j >= H || i >= W) return;
+  const size_t u0 = (size_t)j * (W + 1) + i;
+  out[(size_t)j * W + i] = (fmu[u0] || fmu[u0 + 1] || fmv[(size_t)j * W + i] ||
+                            fmv[(size_t)(j + 1) * W + i]) ? 1 : 0;
+}
+
+__global__ void k_cell_flags(const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv,
+                             int H, int W, uint8_t* __restrict__ flags) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j >= H || i >= W) return;
+  const size_t u0 = (size_t)j * (W + 1) + i;
+  const uint8_t ml = fmu[u0], mr = fmu[u0 + 1];
+  const uint8_t md = fmv[(size_t)j * W + i], mu = fmv[(size_t)(j + 1) * W + i];
+  uint8_t f = (ml || mr || md || mu) ? F_FLUID : 0;
+  if (ml && i > 0) f |= F_LEFT;
+  if (mr && i < W - 1) f |= F_RIGHT;
+  if (md && j > 0) f |= F_DOWN;
+  if (mu && j < H - 1) f |= F_UP;
+  flags[(size_t)j * W + i] = f;
+}
+
+__global__ void k_coarsen_solid(const uint8_t* __restrict__ sf, int H, int W,
+                                uint8_t* __restrict__ sc) {
+  const int Hc = H / 2, Wc = W / 2;
+  const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+  if (J >= Hc || I >= Wc) return;
+  const size_t a = (size_t)(2 * J) * W + 2 * I;
+  sc[(size_t)J * Wc + I] = (sf[a] || sf[a + 1] || sf[a + W] || sf[a + W + 1]) ? 1 : 0;
+}
+
+// ----------------------------------------------------------- advection
+template <typename T>
+__global__ void k_advect_scalar(const T* __restrict__ src, T* __restrict__ dst, int H, int W,
+                                const T* __restrict__ u, int uh, int uw,
+                                const T* __restrict__ v, int vh, int vw,
+                                const uint8_t* __restrict__ fluid, double dt, double inv,
+                                double scale, int apply_scale) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j >= H || i >= W) return;
+  const size_t o = (size_t)j * W + i;
+  if (!fluid[o]) { dst[o] = (T)0; return; }
+  const double px = i + 0.5, py = j + 0.5;
+  const double uu = sample_field(u, uh, uw, px, py - 0.5);
+  const double vv = sample_field(v, vh, vw, px - 0.5, py);
+  const double x = px - (uu * dt * inv);
+  const double y = py - (vv * dt * inv);
+  T r = (T)sample_field(src, H, W, x - 0.5, y - 0.5);
+  if (apply_scale) r = r * (T)scale;
+  dst[o] = r;
+}
+
+template <typename T>
+__device__ __forceinline__ void advect_u_at(const T* __restrict__ us, T* __restrict__ ud, int uh,
+                                            int uw, const T* __restrict__ v, int vh, int vw,
+                                            const uint8_t* __restrict__ fmu, double dt,
+                                            double inv, int j, int i) {
+  const size_t o = (size_t)j * uw + i;
+  if (!fmu[o]) { ud[o] = (T)0; return; }
+  const double px = (double)i, py = j + 0.5;
+  const double uu = (double)us[o];
+  const double vv = sample_field(v, vh, vw, px - 0.5, py);
+  const double x = px - (uu * dt * inv);
+  const double y = py - (vv * dt * inv);
+  ud[o] = (T)sample_field(us, uh, uw, x, y - 0.5);
+}
+
+template <typename T>
+__device__ __forceinline__ void advect_v_at(const T* __restrict__ vs, T* __restrict__ vd, int vh,
+                                            int vw, const T* __restrict__ u, int uh, int uw,
+                                            const uint8_t* __restrict__ fmv, double dt,
+                                            double inv, int j, int i) {
+  const size_t o = (size_t)j * vw + i;
+  if (!fmv[o]) { vd[o] = (T)0; return; }
+  const double px = i + 0.5, py = (double)j;
+  const double uu = sample_field(u, uh, uw, px, py - 0.5);
+  const double vv = (double)vs[o];
+  const double x = px - (uu * dt * inv);
+  const double y = py - (vv * dt * inv);
+  vd[o] = (T)sample_field(vs, vh, vw, x - 0.5, y);
+}
+
+template <typename T>
+__global__ void k_advect_u(const T* __restrict__ us, T* __restrict__ ud, int uh, int uw,
+                           const T* __restrict__ v, int vh, int vw,
+                           const uint8_t* __restrict__ fmu, double dt, double inv) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j < uh && i < uw) advect_u_at(us, ud, uh, uw, v, vh, vw, fmu, dt, inv, j, i);
+}
+
+template <typename T>
+__global__ void k_advect_v(const T* __restrict__ vs, T* __restrict__ vd, int vh, int vw,
+                           const T* __restrict__ u, int uh, int uw,
+                           const uint8_t* __restrict__ fmv, double dt, double inv) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j < vh && i < vw) advect_v_at(vs, vd, vh, vw, u, uh, uw, fmv, dt, inv, j, i);
+}
+
+// u and v faces of an H x W MAC grid in one launch over (H+1) x (W+1).
+template <typename T>
+__global__ void k_advect_velocity(const T* __restrict__ us, T* __restrict__ ud,
+                                  const T* __restrict__ vs, T* __restrict__ vd,
+                                  const uint8_t* __restrict__ fmu,
+                                  const uint8_t* __restrict__ fmv, int H, int W, double dt,
+                                  double inv) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j < H && i <= W) advect_u_at(us, ud, H, W + 1, vs, H + 1, W, fmu, dt, inv, j, i);
+  if (j <= H && i < W) advect_v_at(vs, vd, H + 1, W, us, H, W + 1, fmv, dt, inv, j, i);
+}
+
+// --------------------------------------------------------------- forces
+// forces.py:19-46 / :48-76 / :105.  The force buffers are float32 in the
+// reference, so each force is rounded to float, summed in float, then
+// widened: v += dt * (float)(bf + tf).
+template <typename T>
+__global__ void k_apply_forces(T* __restrict__ v, int vh, int vw,
+                               const T* __restrict__ rho, int H, int W,
+                               const T* __restrict__ temp, int Ht, int Wt, double dt,
+                               double g, double bc, double dc, double Ta) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j >= vh || i >= vw) return;
+  float bf = 0.0f, tf = 0.0f;
+  if (j < H && i < W) {
+    double d;
+    if (j == 0 || j == H - 1) {
+      d = (double)rho[(size_t)j * W + i];
+    } else {
+      const T s = rho[(size_t)(j - 1) * W + i] + rho[(size_t)j * W + i];
+      d = 0.5 * (double)s;
+    }
+    bf = (float)(-dc * d * g);
+  }
+  if (j <= Ht && i < Wt) {
+    double t;
+    if (temp == nullptr) {
+      t = 0.0;
+    } else if (j == 0) {
+      t = (double)temp[i];
+    } else if (j == Ht) {
+      t = (double)temp[(size_t)(Ht - 1) * Wt + i];
+    } else {
+      const T s = temp[(size_t)(j - 1) * Wt + i] + temp[(size_t)j * Wt + i];
+      t = 0.5 * (double)s;
+    }
+    tf = (float)(-bc * (t - Ta));
+  }
+  const float s = bf + tf;
+  const size_t o = (size_t)j * vw + i;
+  v[o] = (T)((double)v[o] + dt * (double)s);
+}
+
+template <typename T>
+__global__ void k_cool(T* __restrict__ t, long n, double ambient, double decay) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  t[k] = (T)(ambient + ((double)t[k] - ambient) * decay);
+}
+
+template <typename T>
+__global__ void k_scale(T* __restrict__ x, long n, T f) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = x[k] * f;
+}
+
+// ----------------------------------------------------------- projection
+// Numba typing: velocity*mask and the difference stay in the array type, the
+// 1/h product and the sum are float64 (inv_cell_size is a Python float).
+template <typename T>
+__global__ void k_divergence(const T* __restrict__ u, const T* __restrict__ v,
+                             T* __restrict__ div, const uint8_t* __restrict__ fmu,
+                             const uint8_t* __restrict__ fmv, int H, int W, double inv) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j >= H || i >= W) return;
+  const size_t a = (size_t)j * (W + 1) + i;
+  const size_t b = (size_t)j * W + i;
+  const T ddu = (u[a + 1] * (T)fmu[a + 1]) - (u[a] * (T)fmu[a]);
+  const T ddv = (v[b + W] * (T)fmv[b + W]) - (v[b] * (T)fmv[b]);
+  const double du = (double)ddu * inv;
+  const double dv = (double)ddv * inv;
+  div[b] = (T)(du + dv);
+}
+
+// pressure_gradient.py:4-37 (+ boundaries.py:7-39 when solid != nullptr).
+template <typename T>
+__global__ void k_pressure_gradient(T* __restrict__ u, T* __restrict__ v,
+                                    const T* __restrict__ p,
+                                    const uint8_t* __restrict__ fmu,
+                                    const uint8_t* __restrict__ fmv,
+                                    const uint8_t* __restrict__ solid, int H, int W,
+                                    double scale) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j < H && i <= W) {
+    const size_t o = (size_t)j * (W + 1) + i;
+    T val;
+    if (fmu[o]) {
+      const int ir = i < W ? i : W - 1;      // open face past the end: clamped
+      const int il = i - 1 < 0 ? W - 1 : i - 1;  // index -1 wraps (Python)
+      const T dp = p[(size_t)j * W + ir] - p[(size_t)j * W + il];
+      val = (T)((double)u[o] - scale * (double)dp);
+    } else {
+      val = (T)0;
+    }
+    if (solid != nullptr) {
+      const bool ls = i > 0 ? solid[(size_t)j * W + i - 1] == 1 : false;
+      const bool rs = i < W ? solid[(size_t)j * W + i] == 1 : false;
+      if (ls || rs) val = (T)0;
+    }
+    u[o] = val;
+  }
+  if (j <= H && i < W) {
+    const size_t o = (size_t)j * W + i;
+    T val;
+    if (fmv[o]) {
+      const int jt = j < H ? j : H - 1;
+      const int jb = j - 1 < 0 ? H - 1 : j - 1;
+      const T dp = p[(size_t)jt * W + i] - p[(size_t)jb * W + i];
+      val = (T)((double)v[o] - scale * (double)dp);
+    } else {
+      val = (T)0;
+    }
+    if (solid != nullptr) {
+      const bool ts = j > 0 ? solid[(size_t)(j - 1) * W + i] == 1 : false;
+      const bool bs = j < H ? solid[(size_t)j * W + i] == 1 : false;
+      if (ts || bs) val = (T)0;
+    }
+    v[o] = val;
+  }
+}
+
+template <typename T>
+__global__ void k_enforce_boundaries(T* __restrict__ u, T* __restrict__ v,
+                                     const uint8_t* __restrict__ solid, int H, int W) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j < H && i <= W) {
+    const bool ls = i > 0 ? solid[(size_t)j * W + i - 1] == 1 : false;
+    const bool rs = i < W ? solid[(size_t)j * W + i] == 1 : false;
+    if (ls || rs) u[(size_t)j * (W + 1) + i] = (T)0;
+  }
+  if (j <= H && i < W) {
+    const bool ts = j > 0 ? solid[(size_t)(j - 1) * W + i] == 1 : false;
+    const bool bs = j < H ? solid[(size_t)j * W + i] == 1 : false;
+    if (ts || bs) v[(size_t)j * W + i] = (T)0;
+  }
+}
+
+// residual.py:6-42, reading the three masks directly.
+template <typename T>
+__global__ void k_residual_masks(const T* __restrict__ p, const T* __restrict__ div,
+                                 const uint8_t* __restrict__ fluid,
+                                 const uint8_t* __restrict__ fmu,
+                                 const uint8_t* __restrict__ fmv, int H, int W,
+                                 double alpha, double beta, T* __restrict__ r) {
+  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  if (j >= H || i >= W) return;
+  const size_t o = (size_t)j * W + i;
+  if (!fluid[o]) { r[o] = (T)0; return; }
+  const size_t a = (size_t)j * (W + 1) + i;
+  const T c = p[o];
+  const T left = fmu[a] ? p[(size_t)j * W + (i - 1 > 0 ? i - 1 : 0)] : c;
+  const T right = fmu[a + 1] ? p[(size_t)j * W + (i + 1 < W - 1 ? i + 1 : W - 1)] : c;
+  const T down = fmv[o] ? p[(size_t)(j - 1 > 0 ? j - 1 : 0) * W + i] : c;
+  const T up = fmv[o + W] ? p[(size_t)(j + 1 < H - 1 ? j + 1 : H - 1) * W + i] : c;
+  // neighbour sum in the array type, alpha/beta terms in float64 (Numba typing)
+  const T nsum = left + right + down + up;
+  r[o] = (T)(((double)nsum + alpha * (double)div[o]) - beta * (double)c);
+}
+
+// ------------------------------------------------------------- emitters
+// emitters.py:339-411 falloffs.
+__device__ __forceinline__ double falloff(double d, double r, int approach) {
+  switch (approach) {
+    case 0: return d < r ? 1.0 : 0.0;
+    case 1: return d >= r ? 0.0 : 1.0 - (d / r);
+    case 2: {
+      if (d >= r) return 0.0;
+      const double q = d / r;
+      return 1.0 - q * q;
+    }
+    case 3: {
+      if (d >= r) return 0.0;
+      const double sigma = r / 3.0;
+      const double q = d / sigma;
+      return exp(-0.5 * (q * q));
+    }
+    default: return 0.0;
+  }
+}
+
+template <typename T>
+__global__ void k_emit(T* __restrict__ f, int H, int W, int x0, int y0, int x1, int y1,
+                       double value, double dt, double px, double py, double radius,
+                       int approach) {
+  const int x = x0 + blockIdx.x * BX + threadIdx.x, y = y0 + blockIdx.y * BY + threadIdx.y;
+  if (x >= x1 || y >= y1) return;
+  const double dx = x - px, dy = y - py;
+  const double d = sqrt(dx * dx + dy * dy);
+  const double fo = falloff(d, radius, approach);
+  const size_t o = (size_t)y * W + x;
+  f[o] = (T)((double)f[o] + value * fo * dt);
+}
+
+template <typename T>
+__global__ void k_box(T* __restrict__ f, int W, int x0, int y0, int x1, int y1, T value,
+                      int mode) {
+  const int x = x0 + blockIdx.x * BX + threadIdx.x, y = y0 + blockIdx.y * BY + threadIdx.y;
+  if (x >= x1 || y >= y1) return;
+  const size_t o = (size_t)y * W + x;
+  f[o] = mode == 0 ? value : f[o] + value;
+}
+
+}  // namespace smk
+
+using namespace smk;
+
+#define SMK_DISPATCH(dtype, KERNEL_CALL_F64, KERNEL_CALL_F32)                 \
+  do {                                                                       \
+    if ((dtype) == SMK_F64) {                                                \
+      KERNEL_CALL_F64;                                                       \
+    } else if ((dtype) == SMK_F32) {                                         \
+      KERNEL_CALL_F32;                                                       \
+    } else {                                                                 \
+      return fail(SMK_EINVAL, "dtype must be SMK_F64 (0) or SMK_F32 (1), got %d", (dtype)); \
+    }                                                                        \
+  } while (0)
+
+extern "C" {
+
+const char* smk_last_error(void) { return g_last_error.c_str(); }
+int smk_abi_version(void) { return 1; }
+
+int smk_face_masks_from_solid(const uint8_t* solid, int H, int W, uint8_t* fmu,
+                              uint8_t* fmv, void* stream) {
+  SMK_REQUIRE(solid && fmu && fmv && H > 0 && W > 0, "smk_face_masks_from_solid: bad arguments");
+  hipLaunchKernelGGL(k_face_masks, grid2d(H + 1, W + 1), dim3(BX, BY), 0, as_stream(stream),
+                     solid, H, W, fmu, fmv);
+  return check_launch("k_face_masks");
+}
+
+int smk_is_fluid(const uint8_t* fmu, const uint8_t* fmv, int H, int W, uint8_t* out,
+                 void* stream) {
+  SMK_REQUIRE(fmu && fmv && out && H > 0 && W > 0, "smk_is_fluid: bad arguments");
+  hipLaunchKernelGGL(k_is_fluid, grid2d(H, W), dim3(BX, BY), 0, as_stream(stream), fmu, fmv,
+                     H, W, out);
+  return check_launch("k_is_fluid");
+}
+
+int smk_cell_flags(const uint8_t* fmu, const uint8_t* fmv, int H, int W, uint8_t* flags,
+                   void* stream) {
+  SMK_REQUIRE(fmu && fmv && flags && H > 0 && W > 0, "smk_cell_flags: bad arguments");
+  hipLaunchKernelGGL(k_cell_flags, grid2d(H, W), dim3(BX, BY), 0, as_stream(stream), fmu, fmv,
+                     H, W, flags);
+  return check_launch("k_cell_flags");
+}
+
+int smk_coarsen_solid(const uint8_t* sf, int H, int W, uint8_t* sc, void* stream) {
+  SMK_REQUIRE(sf && sc && H > 1 && W > 1, "smk_coarsen_solid: bad arguments");
+  hipLaunchKernelGGL(k_coarsen_solid, grid2d(H / 2, W / 2), dim3(BX, BY), 0, as_stream(stream),
+                     sf, H, W, sc);
+  return check_launch("k_coarsen_solid");
+}
+
+int smk_advect_scalar(const void* src, void* dst, int H, int W, const void* u, int uh, int uw,
+                      const void* v, int vh, int vw, const uint8_t* is_fluid, double dt,
+                      double cell_size, double scale, int dtype, void* stream) {
+  SMK_REQUIRE(src && dst && u && v && is_fluid, "smk_advect_scalar: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0 && uh > 0 && uw > 0 && vh > 0 && vw > 0,
+              "smk_advect_scalar: bad shape");
+  SMK_REQUIRE(cell_size != 0.0, "smk_advect_scalar: cell_size must be non-zero");
+  const double inv = 1.0 / cell_size;
+  const int aps = scale != 1.0;
+  auto g = grid2d(H, W);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_advect_scalar<double>, g, dim3(BX, BY), 0, s,
+                                  (const double*)src, (double*)dst, H, W, (const double*)u, uh,
+                                  uw, (const double*)v, vh, vw, is_fluid, dt, inv, scale, aps),
+               hipLaunchKernelGGL(k_advect_scalar<float>, g, dim3(BX, BY), 0, s,
+                                  (const float*)src, (float*)dst, H, W, (const float*)u, uh, uw,
+                                  (const float*)v, vh, vw, is_fluid, dt, inv, scale, aps));
+  return check_launch("k_advect_scalar");
+}
+
+int smk_advect_u(const void* us, void* ud, int uh, int uw, const void* v, int vh, int vw,
+                 const uint8_t* fmu, double dt, double cell_size, int dtype, void* stream) {
+  SMK_REQUIRE(us && ud && v && fmu, "smk_advect_u: null pointer");
+  SMK_REQUIRE(uh > 0 && uw > 0 && vh > 0 && vw > 0, "smk_advect_u: bad shape");
+  SMK_REQUIRE(cell_size != 0.0, "smk_advect_u: cell_size must be non-zero");
+  const double inv = 1.0 / cell_size;
+  auto g = grid2d(uh, uw);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_advect_u<double>, g, dim3(BX, BY), 0, s, (const double*)us,
+                                  (double*)ud, uh, uw, (const double*)v, vh, vw, fmu, dt, inv),
+               hipLaunchKernelGGL(k_advect_u<float>, g, dim3(BX, BY), 0, s, (const float*)us,
+                                  (float*)ud, uh, uw, (const float*)v, vh, vw, fmu, dt, inv));
+  return check_launch("k_advect_u");
+}
+
+int smk_advect_v(const void* vs, void* vd, int vh, int vw, const void* u, int uh, int uw,
+                 const uint8_t* fmv, double dt, double cell_size, int dtype, void* stream) {
+  SMK_REQUIRE(vs && vd && u && fmv, "smk_advect_v: null pointer");
+  SMK_REQUIRE(uh > 0 && uw > 0 && vh > 0 && vw > 0, "smk_advect_v: bad shape");
+  SMK_REQUIRE(cell_size != 0.0, "smk_advect_v: cell_size must be non-zero");
+  const double inv = 1.0 / cell_size;
+  auto g = grid2d(vh, vw);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_advect_v<double>, g, dim3(BX, BY), 0, s, (const double*)vs,
+                                  (double*)vd, vh, vw, (const double*)u, uh, uw, fmv, dt, inv),
+               hipLaunchKernelGGL(k_advect_v<float>, g, dim3(BX, BY), 0, s, (const float*)vs,
+                                  (float*)vd, vh, vw, (const float*)u, uh, uw, fmv, dt, inv));
+  return check_launch("k_advect_v");
+}
+
+int smk_advect_velocity(const void* us, void* ud, const void* vs, void* vd, const uint8_t* fmu,
+                        const uint8_t* fmv, int H, int W, double dt, double cell_size, int dtype,
+                        void* stream) {
+  SMK_REQUIRE(us && ud && vs && vd && fmu && fmv, "smk_advect_velocity: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0, "smk_advect_velocity: bad shape");
+  SMK_REQUIRE(cell_size != 0.0, "smk_advect_velocity: cell_size must be non-zero");
+  const double inv = 1.0 / cell_size;
+  auto g = grid2d(H + 1, W + 1);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_advect_velocity<double>, g, dim3(BX, BY), 0, s,
+                                  (const double*)us, (double*)ud, (const double*)vs,
+                                  (double*)vd, fmu, fmv, H, W, dt, inv),
+               hipLaunchKernelGGL(k_advect_velocity<float>, g, dim3(BX, BY), 0, s,
+                                  (const float*)us, (float*)ud, (const float*)vs, (float*)vd,
+                                  fmu, fmv, H, W, dt, inv));
+  return check_launch("k_advect_velocity");
+}
+
+int smk_apply_forces(void* v, int vh, int vw, const void* density, int H, int W,
+                     const void* temperature, int Ht, int Wt, double dt, double gravity,
+                     double buoyancy_coefficient, double density_coefficient,
+                     double ambient_temperature, int dtype, void* stream) {
+  SMK_REQUIRE(v && density, "smk_apply_forces: null pointer");
+  SMK_REQUIRE(vh > 0 && vw > 0 && H > 0 && W > 0, "smk_apply_forces: bad shape");
+  // forces.py:101-105 indexes both (H+1, W) force buffers with v's indices
+  SMK_REQUIRE(vh <= H + 1 && vw <= W, "smk_apply_forces: v larger than the force buffer");
+  if (temperature == nullptr) { Ht = H; Wt = W; }
+  SMK_REQUIRE(vh <= Ht + 1 && vw <= Wt, "smk_apply_forces: v larger than the temperature force");
+  auto g = grid2d(vh, vw);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_apply_forces<double>, g, dim3(BX, BY), 0, s, (double*)v, vh,
+                                  vw, (const double*)density, H, W, (const double*)temperature,
+                                  Ht, Wt, dt, gravity, buoyancy_coefficient, density_coefficient,
+                                  ambient_temperature),
+               hipLaunchKernelGGL(k_apply_forces<float>, g, dim3(BX, BY), 0, s, (float*)v, vh,
+                                  vw, (const float*)density, H, W, (const float*)temperature, Ht,
+                                  Wt, dt, gravity, buoyancy_coefficient, density_coefficient,
+                                  ambient_temperature));
+  return check_launch("k_apply_forces");
+}
+
+int smk_cool_temperature(void* T, int H, int W, double ambient, double decay, int dtype,
+                         void* stream) {
+  SMK_REQUIRE(T && H > 0 && W > 0, "smk_cool_temperature: bad arguments");
+  const long n = (long)H * W;
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_cool<double>, dim3(cdiv(n, 256)), dim3(256), 0, s,
+                                  (double*)T, n, ambient, decay),
+               hipLaunchKernelGGL(k_cool<float>, dim3(cdiv(n, 256)), dim3(256), 0, s, (float*)T,
+                                  n, ambient, decay));
+  return check_launch("k_cool");
+}
+
+int smk_scale(void* x, long n, double factor, int dtype, void* stream) {
+  SMK_REQUIRE(x && n >= 0, "smk_scale: bad arguments");
+  if (n == 0) return SMK_OK;
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_scale<double>, dim3(cdiv(n, 256)), dim3(256), 0, s,
+                                  (double*)x, n, factor),
+               hipLaunchKernelGGL(k_scale<float>, dim3(cdiv(n, 256)), dim3(256), 0, s,
+                                  (float*)x, n, (float)factor));
+  return check_launch("k_scale");
+}
+
+int smk_divergence(const void* u, const void* v, void* div, const uint8_t* fmu,
+                   const uint8_t* fmv, int H, int W, double cell_size, int dtype, void* stream) {
+  SMK_REQUIRE(u && v && div && fmu && fmv, "smk_divergence: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0 && cell_size != 0.0, "smk_divergence: bad arguments");
+  auto g = grid2d(H, W);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_divergence<double>, g, dim3(BX, BY), 0, s, (const double*)u,
+                                  (const double*)v, (double*)div, fmu, fmv, H, W,
+                                  1.0 / cell_size),
+               hipLaunchKernelGGL(k_divergence<float>, g, dim3(BX, BY), 0, s, (const float*)u,
+                                  (const float*)v, (float*)div, fmu, fmv, H, W,
+                                  1.0 / cell_size));
+  return check_launch("k_divergence");
+}
+
+int smk_pressure_gradient(void* u, void* v, const void* p, const uint8_t* fmu,
+                          const uint8_t* fmv, const uint8_t* solid, int H, int W,
+                          double fluid_density, double cell_size, double dt, int dtype,
+                          void* stream) {
+  SMK_REQUIRE(u && v && p && fmu && fmv, "smk_pressure_gradient: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0, "smk_pressure_gradient: bad shape");
+  const double scale = dt / (fluid_density * cell_size);
+  auto g = grid2d(H + 1, W + 1);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_pressure_gradient<double>, g, dim3(BX, BY), 0, s, (double*)u,
+                                  (double*)v, (const double*)p, fmu, fmv, solid, H, W, scale),
+               hipLaunchKernelGGL(k_pressure_gradient<float>, g, dim3(BX, BY), 0, s, (float*)u,
+                                  (float*)v, (const float*)p, fmu, fmv, solid, H, W,
+                                  scale));
+  return check_launch("k_pressure_gradient");
+}
+
+int smk_enforce_boundaries(void* u, void* v, const uint8_t* solid, int H, int W, int dtype,
+                           void* stream) {
+  SMK_REQUIRE(u && v && solid && H > 0 && W > 0, "smk_enforce_boundaries: bad arguments");
+  auto g = grid2d(H + 1, W + 1);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_enforce_boundaries<double>, g, dim3(BX, BY), 0, s,
+                                  (double*)u, (double*)v, solid, H, W),
+               hipLaunchKernelGGL(k_enforce_boundaries<float>, g, dim3(BX, BY), 0, s,
+                                  (float*)u, (float*)v, solid, H, W));
+  return check_launch("k_enforce_boundaries");
+}
+
+int smk_residual(const void* p, const void* div, const uint8_t* is_fluid, const uint8_t* fmu,
+                 const uint8_t* fmv, int H, int W, double alpha, double reciprocal_beta, void* r,
+                 int dtype, void* stream) {
+  SMK_REQUIRE(p && div && is_fluid && fmu && fmv && r, "smk_residual: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0 && reciprocal_beta != 0.0, "smk_residual: bad arguments");
+  const double beta = 1.0 / reciprocal_beta;
+  auto g = grid2d(H, W);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_residual_masks<double>, g, dim3(BX, BY), 0, s,
+                                  (const double*)p, (const double*)div, is_fluid, fmu, fmv, H, W,
+                                  alpha, beta, (double*)r),
+               hipLaunchKernelGGL(k_residual_masks<float>, g, dim3(BX, BY), 0, s,
+                                  (const float*)p, (const float*)div, is_fluid, fmu, fmv, H, W,
+                                  alpha, beta, (float*)r));
+  return check_launch("k_residual_masks");
+}
+
+int smk_emit_scalar(void* field, int H, int W, double value, double dt, double pos_x,
+                    double pos_y, double radius, int approach, int dtype, void* stream) {
+  SMK_REQUIRE(field && H > 0 && W > 0, "smk_emit_scalar: bad arguments");
+  SMK_REQUIRE(approach >= 0 && approach <= 3, "smk_emit_scalar: unknown approach %d", approach);
+  // Cells with distance >= radius get falloff 0 and are left unchanged, so
+  // only the disc's bounding box is visited (emitters.py:52-56 does the same).
+  double fx0 = floor(pos_x - radius) - 1.0, fx1 = ceil(pos_x + radius) + 2.0;
+  double fy0 = floor(pos_y - radius) - 1.0, fy1 = ceil(pos_y + radius) + 2.0;
+  if (!(fx0 == fx0) || !(fy0 == fy0) || !(fx1 == fx1) || !(fy1 == fy1)) return SMK_OK;
+  const int x0 = (int)fmax(0.0, fmin((double)W, fx0)), x1 = (int)fmax(0.0, fmin((double)W, fx1));
+  const int y0 = (int)fmax(0.0, fmin((double)H, fy0)), y1 = (int)fmax(0.0, fmin((double)H, fy1));
+  if (x1 <= x0 || y1 <= y0) return SMK_OK;
+  dim3 g(cdiv(x1 - x0, BX), cdiv(y1 - y0, BY));
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_emit<double>, g, dim3(BX, BY), 0, s, (double*)field, H, W, x0,
+                                  y0, x1, y1, value, dt, pos_x, pos_y, radius, approach),
+               hipLaunchKernelGGL(k_emit<float>, g, dim3(BX, BY), 0, s, (float*)field, H, W, x0,
+                                  y0, x1, y1, value, dt, pos_x, pos_y, radius, approach));
+  return check_launch("k_emit");
+}
+
+int smk_box_source(void* field, int H, int W, int x0, int y0, int x1, int y1, double value,
+                   int mode, int dtype, void* stream) {
+  SMK_REQUIRE(field && H > 0 && W > 0, "smk_box_source: bad arguments");
+  SMK_REQUIRE(mode == 0 || mode == 1, "smk_box_source: mode must be 0 (set) or 1 (add)");
+  x0 = x0 < 0 ? 0 : x0; y0 = y0 < 0 ? 0 : y0;
+  x1 = x1 > W ? W : x1; y1 = y1 > H ? H : y1;
+  if (x1 <= x0 || y1 <= y0) return SMK_OK;
+  dim3 g(cdiv(x1 - x0, BX), cdiv(y1 - y0, BY));
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_box<double>, g, dim3(BX, BY), 0, s, (double*)field, W, x0, y0,
+                                  x1, y1, value, mode),
+               hipLaunchKernelGGL(k_box<float>, g, dim3(BX, BY), 0, s, (float*)field, W, x0, y0,
+                                  x1, y1, (float)value, mode));
+  return check_launch("k_box");
+}
+
+}  // extern "C"

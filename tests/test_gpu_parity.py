@@ -1,0 +1,308 @@
+"""HIP library vs the oracle on seeded inputs (float64: bit-identical
+results expected, since both evaluate the reference's expressions in the same
+order without FMA contraction).  Shapes include sizes that are not multiples
+of the kernels' tile sizes, masks include random obstacles."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16), (37, 53), (64, 96), (130, 70)]
+
+
+def masks_for(H, W, kind, seed=0):
+    solid = np.zeros((H, W), dtype=bool)
+    if kind in ("ring", "obstacles"):
+        solid[0, :] = solid[-1, :] = True
+        solid[:, 0] = solid[:, -1] = True
+    if kind == "obstacles":
+        solid |= np.random.default_rng(seed).random((H, W)) < 0.15
+    fmu = np.zeros((H, W + 1), dtype=bool)
+    fmv = np.zeros((H + 1, W), dtype=bool)
+    fmu[:, 1:-1] = ~(solid[:, :-1] | solid[:, 1:])
+    fmv[1:-1, :] = ~(solid[:-1, :] | solid[1:, :])
+    fl = fmu[:, :-1] | fmu[:, 1:] | fmv[:-1, :] | fmv[1:, :]
+    return solid, fmu, fmv, fl
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import cfd_amd.operators  # noqa: F401
+    import cfd_amd.solvers  # noqa: F401
+    from cfd_amd.backends.hip import operators, solvers
+    return operators, solvers
+
+
+def fields(H, W, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal((H, W + 1)) * scale, rng.standard_normal((H + 1, W)) * scale,
+            rng.random((H, W)), rng.random((H, W)))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("kind", ["open", "ring", "obstacles"])
+def test_advection_bitwise(hip, shape, kind):
+    ops, _ = hip
+    H, W = shape
+    solid, fmu, fmv, fl = masks_for(H, W, kind, seed=H)
+    u, v, rho, _ = fields(H, W, seed=H * W, scale=40.0)  # large CFL: clamped backtraces
+    dt, h = 0.005, 1.0 / H
+    # scalar
+    ref = np.zeros((H, W))
+    O.advect_scalar_field(rho, ref, u, v, fl, dt, h)
+    out = dev(np.full((H, W), np.nan))
+    ops.advect_scalar_field(dev(rho), out, dev(u), dev(v), dev(fl), dt, h)
+    np.testing.assert_array_equal(host(out), ref)
+    # u and v faces, separately and fused
+    ru, rv = np.zeros_like(u), np.zeros_like(v)
+    O.advect_velocity_u(u, ru, v, fmu, dt, h)
+    O.advect_velocity_v(v, rv, u, fmv, dt, h)
+    ou, ov = dev(np.full_like(u, np.nan)), dev(np.full_like(v, np.nan))
+    ops.advect_velocity_u(dev(u), ou, dev(v), dev(fmu), dt, h)
+    ops.advect_velocity_v(dev(v), ov, dev(u), dev(fmv), dt, h)
+    np.testing.assert_array_equal(host(ou), ru)
+    np.testing.assert_array_equal(host(ov), rv)
+    fu, fv = dev(np.full_like(u, np.nan)), dev(np.full_like(v, np.nan))
+    ops.advect_velocity(dev(u), fu, dev(v), fv, dev(fmu), dev(fmv), dt, h)
+    np.testing.assert_array_equal(host(fu), ru)
+    np.testing.assert_array_equal(host(fv), rv)
+
+
+def test_advection_cell_shaped_velocity(hip):
+    """The reference's advection tests pass H x W velocity arrays; the
+    samplers clamp to each array's own shape (advection.py:24)."""
+    ops, _ = hip
+    H, W = 24, 40
+    rng = np.random.default_rng(3)
+    u, v, src = rng.standard_normal((H, W)) * 3, rng.standard_normal((H, W)) * 3, rng.random((H, W))
+    fl = np.ones((H, W), dtype=bool)
+    ref = np.zeros((H, W))
+    O.advect_scalar_field(src, ref, u, v, fl, 0.7, 1.0)
+    out = dev(np.zeros((H, W)))
+    ops.advect_scalar_field(dev(src), out, dev(u), dev(v), dev(fl), 0.7, 1.0)
+    np.testing.assert_array_equal(host(out), ref)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("with_temperature", [False, True])
+def test_forces_bitwise(hip, shape, with_temperature):
+    ops, _ = hip
+    H, W = shape
+    _, v, rho, T = fields(H, W, seed=7 + H)
+    args = (0.016, -0.5, 5.0, 1.0, 0.25)
+    ref = v.copy()
+    O.apply_forces(ref, rho, T if with_temperature else None, *args)
+    out = dev(v)
+    ops.apply_forces(out, dev(rho), dev(T) if with_temperature else None, *args)
+    np.testing.assert_array_equal(host(out), ref)
+
+
+def test_cool_bitwise(hip):
+    ops, _ = hip
+    T = np.random.default_rng(1).random((33, 65))
+    ref = T.copy()
+    O.cool_temperature(ref, 0.3, 0.5, 0.016)
+    out = dev(T)
+    ops.cool_temperature(out, 0.3, 0.5, 0.016)
+    np.testing.assert_array_equal(host(out), ref)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("kind", ["open", "ring", "obstacles"])
+def test_projection_ops_bitwise(hip, shape, kind):
+    ops, _ = hip
+    H, W = shape
+    solid, fmu, fmv, fl = masks_for(H, W, kind, seed=W)
+    u, v, p, d = fields(H, W, seed=H + 3 * W)
+    h = 1.0 / H
+    # divergence
+    ref = np.zeros((H, W))
+    O.compute_divergence(u, v, ref, fmu, fmv, h)
+    out = dev(np.full((H, W), np.nan))
+    ops.compute_divergence(dev(u), dev(v), out, dev(fmu), dev(fmv), h)
+    np.testing.assert_array_equal(host(out), ref)
+    # gradient, then boundaries (separately and fused)
+    ru, rv = u.copy(), v.copy()
+    O.apply_pressure_gradient(ru, rv, p, fmu, fmv, 1.0, h, 0.016)
+    ou, ov = dev(u), dev(v)
+    ops.apply_pressure_gradient(ou, ov, dev(p), dev(fmu), dev(fmv), 1.0, h, 0.016)
+    np.testing.assert_array_equal(host(ou), ru)
+    np.testing.assert_array_equal(host(ov), rv)
+    O.enforce_boundary_conditions(ru, rv, solid)
+    ops.enforce_boundary_conditions(ou, ov, dev(solid))
+    np.testing.assert_array_equal(host(ou), ru)
+    np.testing.assert_array_equal(host(ov), rv)
+    fu, fv = dev(u), dev(v)
+    ops.apply_pressure_gradient(fu, fv, dev(p), dev(fmu), dev(fmv), 1.0, h, 0.016,
+                                solid_mask=dev(solid))
+    np.testing.assert_array_equal(host(fu), ru)
+    np.testing.assert_array_equal(host(fv), rv)
+    # residual
+    alpha = O.calculate_alpha(h, 1.0, 0.016)
+    rr = np.zeros((H, W))
+    O.compute_residual(p, d, fl, fmu, fmv, alpha, 0.25, rr)
+    orr = dev(np.full((H, W), np.nan))
+    ops.compute_residual(dev(p), dev(d), dev(fl), dev(fmu), dev(fmv), alpha, 0.25, orr)
+    np.testing.assert_array_equal(host(orr), rr)
+
+
+@pytest.mark.parametrize("approach", ["hard_cutoff", "linear", "quadratic", "gaussian"])
+def test_emitter_matches_oracle(hip, approach):
+    ops, _ = hip
+    H, W = 48, 64
+    f = np.random.default_rng(5).random((H, W))
+    ref = f.copy()
+    O.emit_scalar(ref, 2.5, 0.016, 20.3, 30.7, 7.5, approach)
+    out = dev(f)
+    ops.emit_scalar(out, 2.5, 0.016, 20.3, 30.7, 7.5, approach)
+    # device exp may differ from glibc's in the last ulp (gaussian only)
+    if approach == "gaussian":
+        np.testing.assert_allclose(host(out), ref, rtol=1e-15, atol=0)
+    else:
+        np.testing.assert_array_equal(host(out), ref)
+
+
+def test_box_source(hip):
+    ops, _ = hip
+    f = np.random.default_rng(6).random((40, 50))
+    out = dev(f)
+    ops.box_source(out, 10, 5, 26, 13, 1.25)
+    ref = f.copy()
+    ref[5:13, 10:26] = 1.25
+    np.testing.assert_array_equal(host(out), ref)
+    ops.box_source(out, -3, 30, 60, 50, 0.5, mode="add")
+    ref[30:40, 0:50] += 0.5
+    np.testing.assert_array_equal(host(out), ref)
+
+
+# ---------------------------------------------------------------- solvers
+def rhs_for(H, W, fmu, fmv, seed):
+    """A consistent right-hand side: the divergence of a random velocity."""
+    u, v, _, _ = fields(H, W, seed)
+    d = np.zeros((H, W))
+    O.compute_divergence(u, v, d, fmu, fmv, 1.0 / H)
+    return d
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("iterations", [1, 2, 7, 40])
+def test_jacobi_bitwise(hip, shape, iterations):
+    _, sol = hip
+    H, W = shape
+    _, fmu, fmv, _ = masks_for(H, W, "obstacles", seed=1)
+    d = rhs_for(H, W, fmu, fmv, 11)
+    p0 = np.random.default_rng(2).random((H, W))
+    ref = p0.copy()
+    O.jacobi_pressure_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, iterations)
+    out = dev(p0)
+    sol.jacobi_pressure_solver(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H, 0.005, iterations)
+    np.testing.assert_array_equal(host(out), ref)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("iterations,omega", [(1, 1.0), (3, 1.7), (10, 0.0), (9, 1.9)])
+def test_rbsor_bitwise(hip, shape, iterations, omega):
+    _, sol = hip
+    H, W = shape
+    _, fmu, fmv, _ = masks_for(H, W, "obstacles", seed=2)
+    d = rhs_for(H, W, fmu, fmv, 12)
+    p0 = np.random.default_rng(3).random((H, W))
+    ref = p0.copy()
+    O.red_black_sor_gauss_seidel_pressure_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005,
+                                                 iterations, omega)
+    out = dev(p0)
+    sol.red_black_sor_gauss_seidel_pressure_solver(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H,
+                                                   0.005, iterations, omega)
+    np.testing.assert_array_equal(host(out), ref)
+
+
+def test_rbsor_tolerance_loop(hip):
+    _, sol = hip
+    H, W = 64, 64
+    _, fmu, fmv, _ = masks_for(H, W, "open")
+    d = rhs_for(H, W, fmu, fmv, 13)
+    ref = np.zeros((H, W))
+    n_ref, rel_ref = O.rbsor_solve_tol(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, 1.9, 1e-6, 20, 5000)
+    out = dev(np.zeros((H, W)))
+    sol.red_black_sor_gauss_seidel_pressure_solver(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H,
+                                                   0.005, 5000, 1.9, tolerance=1e-6,
+                                                   check_every=20)
+    info = dict(sol.LAST_SOLVE_INFO)
+    assert info["iterations"] == n_ref
+    assert info["converged"] == 1 and info["rel_residual"] <= 1e-6
+    np.testing.assert_array_equal(host(out), ref)
+
+
+MG_CASES = [
+    # reference defaults: Jacobi smoother, 4 levels, 2 cycles, coarsest 30
+    dict(v_cycles=2, smoother_type="jacobi", num_levels=4, smoother_iterations=2,
+         coarsest_iterations=30),
+    # RB-SOR smoother with the reference's automatic omega per level
+    dict(v_cycles=3, smoother_type="rb_sor_gauss_seidel", num_levels=5, smoother_iterations=2,
+         coarsest_iterations=50),
+    # RB-GS (omega=1) with the rediscretised coarse scaling
+    dict(v_cycles=3, smoother_type="rb_sor_gauss_seidel", smoother_omega=1.0, num_levels=6,
+         smoother_iterations=2, coarsest_iterations=50, coarse_scale="rediscretized"),
+    # odd smoothing counts and many passes per level
+    dict(v_cycles=2, smoother_type="rb_sor_gauss_seidel", smoother_omega=1.3, num_levels=3,
+         smoother_iterations=5, coarsest_iterations=7),
+    dict(v_cycles=2, smoother_type="jacobi", num_levels=3, smoother_iterations=9,
+         coarsest_iterations=11),
+]
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (48, 80), (96, 40), (128, 128)])
+@pytest.mark.parametrize("case", range(len(MG_CASES)))
+@pytest.mark.parametrize("kind", ["open", "obstacles"])
+def test_multigrid_bitwise(hip, shape, case, kind):
+    _, sol = hip
+    H, W = shape
+    _, fmu, fmv, _ = masks_for(H, W, kind, seed=4)
+    d = rhs_for(H, W, fmu, fmv, 14)
+    kw = MG_CASES[case]
+    ref = np.zeros((H, W))
+    O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **kw)
+    out = dev(np.zeros((H, W)))
+    mg = sol.MultigridSolver()
+    mg(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H, 0.005, **kw)
+    np.testing.assert_array_equal(host(out), ref)
+
+
+def test_multigrid_tolerance_and_mixed(hip):
+    _, sol = hip
+    H = W = 128
+    _, fmu, fmv, _ = masks_for(H, W, "open")
+    d = rhs_for(H, W, fmu, fmv, 15)
+    kw = dict(v_cycles=100, smoother_type="rb_sor_gauss_seidel", smoother_omega=1.0,
+              num_levels=4, smoother_iterations=2, coarsest_iterations=50,
+              coarse_scale="rediscretized", tolerance=1e-8)
+    ref = np.zeros((H, W))
+    n_ref, rel_ref = O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **kw)
+    mg = sol.MultigridSolver()
+    out = dev(np.zeros((H, W)))
+    mg(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H, 0.005, **kw)
+    info = dict(sol.LAST_SOLVE_INFO)
+    assert info["iterations"] == n_ref and info["converged"] == 1
+    np.testing.assert_array_equal(host(out), ref)
+    # FP32 V-cycles inside FP64 defect correction reach the same tolerance
+    mix = dev(np.zeros((H, W)))
+    mg(mix, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H, 0.005, precision="mixed", **kw)
+    info = dict(sol.LAST_SOLVE_INFO)
+    assert info["converged"] == 1 and info["rel_residual"] <= 1e-8
+    assert info["refinements"] >= 1
+    # both solutions solve the same singular system: compare modulo the constant
+    a, b = host(mix), ref
+    a, b = a - a.mean(), b - b.mean()
+    assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(b)
