@@ -1,0 +1,428 @@
+"""Pins of the oracle that do not retype its loops: dense linear algebra,
+closed forms and invariants, each chosen so that a plausible transcription
+mistake (wrong ping-pong, colour order, omega, mirror rule, restriction
+scaling, float32 force rounding, staggered offset) fails.
+
+The Poisson operator is assembled as a dense matrix A (A p = 4p - sum of
+neighbours, a closed face mirroring the cell itself), so the Jacobi, RB-SOR,
+Gauss-Seidel and multigrid restatements are checked as matrix iterations,
+splittings and operator compositions rather than as stencil loops.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle as O
+
+RTOL = 1e-12
+
+
+def masks(H, W, kind, seed=0):
+    solid = np.zeros((H, W), dtype=bool)
+    if kind in ("ring", "obstacles"):
+        solid[0, :] = solid[-1, :] = True
+        solid[:, 0] = solid[:, -1] = True
+    if kind == "obstacles":
+        solid |= np.random.default_rng(seed).random((H, W)) < 0.2
+    fmu = np.zeros((H, W + 1), dtype=bool)
+    fmv = np.zeros((H + 1, W), dtype=bool)
+    fmu[:, 1:-1] = ~(solid[:, :-1] | solid[:, 1:])
+    fmv[1:-1, :] = ~(solid[:-1, :] | solid[1:, :])
+    fl = fmu[:, :-1] | fmu[:, 1:] | fmv[:-1, :] | fmv[1:, :]
+    return solid, fmu, fmv, fl
+
+
+def adjacency(fmu, fmv):
+    """N[k, n] = number of times cell n enters cell k's neighbour sum."""
+    H, W = fmu.shape[0], fmu.shape[1] - 1
+    N = np.zeros((H * W, H * W))
+    for j in range(H):
+        for i in range(W):
+            k = j * W + i
+            for open_, jj, ii in ((fmu[j, i], j, i - 1), (fmu[j, i + 1], j, i + 1),
+                                  (fmv[j, i], j - 1, i), (fmv[j + 1, i], j + 1, i)):
+                inside = 0 <= jj < H and 0 <= ii < W
+                N[k, jj * W + ii if (open_ and inside) else k] += 1
+    return N
+
+
+def close(a, b, rtol=RTOL):
+    scale = max(np.abs(b).max(), 1e-300)
+    assert np.abs(a - b).max() <= rtol * scale, np.abs(a - b).max() / scale
+
+
+@pytest.fixture(params=[("open", 6, 8), ("ring", 8, 8), ("obstacles", 9, 7), ("obstacles", 12, 10)])
+def grid(request):
+    kind, H, W = request.param
+    solid, fmu, fmv, fl = masks(H, W, kind, seed=H * W)
+    rng = np.random.default_rng(H + W)
+    return dict(H=H, W=W, solid=solid, fmu=fmu, fmv=fmv, fl=fl, N=adjacency(fmu, fmv),
+                p=rng.standard_normal((H, W)), div=rng.standard_normal((H, W)))
+
+
+ALPHA = O.calculate_alpha(0.25, 1.3, 0.02)
+
+
+def test_alpha_and_omega_closed_forms():
+    assert O.calculate_alpha(0.25, 1.3, 0.02) == -(0.25 * 0.25) * 1.3 / 0.02
+    for n in (4, 16, 100, 512, 2048):
+        w = 2.0 / (1.0 + math.sin(math.pi / n))
+        assert O.calculate_optimal_omega(n) == min(w, 1.95)
+    assert O.calculate_optimal_omega(2048) == 1.95
+
+
+def test_residual_is_b_minus_Ap(grid):
+    g = grid
+    p, fl = g["p"].ravel(), g["fl"].ravel()
+    b = ALPHA * g["div"].ravel()
+    A = 4 * np.eye(p.size) - g["N"]
+    expected = np.where(fl, b - A @ p, 0.0)
+    r = np.zeros_like(g["p"])
+    O.compute_residual(g["p"], g["div"], g["fl"], g["fmu"], g["fmv"], ALPHA, 0.25, r)
+    close(r.ravel(), expected)
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 16])
+def test_jacobi_is_matrix_iteration(grid, k):
+    g = grid
+    fl = g["fl"].ravel()
+    b = ALPHA * g["div"].ravel()
+    p = g["p"].ravel().copy()
+    for _ in range(k):
+        p = np.where(fl, (g["N"] @ p + b) / 4.0, p)
+    out = g["p"].copy()
+    O.jacobi_pressure_solver(out, g["div"], g["fmu"], g["fmv"], 1.3, 0.25, 0.02, k)
+    close(out.ravel(), p)
+
+
+@pytest.mark.parametrize("k,omega", [(1, 1.0), (3, 1.5), (6, 1.95), (2, 0.7)])
+def test_rbsor_is_coloured_splitting(grid, k, omega):
+    g = grid
+    H, W = g["H"], g["W"]
+    jj, ii = np.mgrid[0:H, 0:W]
+    colour = ((ii + jj) % 2).ravel()
+    fl = g["fl"].ravel()
+    b = ALPHA * g["div"].ravel()
+    p = g["p"].ravel().copy()
+    for _ in range(k):
+        for c in (0, 1):  # colour 0 ((i+j) even) first
+            sel = fl & (colour == c)
+            new = (1 - omega) * p + omega * (g["N"] @ p + b) / 4.0
+            p = np.where(sel, new, p)
+    out = g["p"].copy()
+    O.red_black_sor_gauss_seidel_pressure_solver(out, g["div"], g["fmu"], g["fmv"], 1.3, 0.25,
+                                                 0.02, k, omega)
+    close(out.ravel(), p)
+
+
+def test_rbsor_automatic_omega(grid):
+    g = grid
+    a, b = g["p"].copy(), g["p"].copy()
+    O.red_black_sor_gauss_seidel_pressure_solver(a, g["div"], g["fmu"], g["fmv"], 1.3, 0.25,
+                                                 0.02, 3, 0.0)
+    w = min(2.0 / (1.0 + math.sin(math.pi / max(g["H"], g["W"]))), 1.95)
+    O.red_black_sor_gauss_seidel_pressure_solver(b, g["div"], g["fmu"], g["fmv"], 1.3, 0.25,
+                                                 0.02, 3, w)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_gauss_seidel_is_triangular_splitting(grid, k):
+    """Lexicographic GS: (4I - L) p_new = (U + M) p_old + b on fluid rows,
+    L/U the adjacency to earlier/later cells, M the mirrored self terms."""
+    g = grid
+    n = g["H"] * g["W"]
+    fl = g["fl"].ravel()
+    N = g["N"]
+    L = np.tril(N, -1)
+    U = np.triu(N, 1)
+    M = np.diag(np.diag(N))
+    b = ALPHA * g["div"].ravel()
+    Afl = np.where(fl[:, None], 4 * np.eye(n) - L, np.eye(n))
+    p = g["p"].ravel().copy()
+    for _ in range(k):
+        rhs = np.where(fl, (U + M) @ p + b, p)
+        p = scipy.linalg.solve_triangular(Afl, rhs, lower=True)
+    out = g["p"].copy()
+    O.gauss_seidel_pressure_solver(out, g["div"], g["fmu"], g["fmv"], 1.3, 0.25, 0.02, k)
+    close(out.ravel(), p, 1e-11)
+
+
+def test_rbsor_converges_to_least_squares_solution():
+    """On a consistent system the iteration's fixed point solves A p = b
+    modulo the constant null vector (closed walls)."""
+    H, W = 10, 12
+    _, fmu, fmv, fl = masks(H, W, "open")
+    N = adjacency(fmu, fmv)
+    A = 4 * np.eye(H * W) - N
+    rng = np.random.default_rng(1)
+    b = A @ rng.standard_normal(H * W)  # in the range of A
+    div = (b / ALPHA).reshape(H, W)
+    p = np.zeros((H, W))
+    O.red_black_sor_gauss_seidel_pressure_solver(p, div, fmu, fmv, 1.3, 0.25, 0.02, 4000, 1.8)
+    x = np.linalg.lstsq(A, ALPHA * div.ravel(), rcond=None)[0]
+    d = p.ravel() - x
+    assert np.abs(d - d.mean()).max() <= 1e-9 * np.abs(x).max()
+
+
+# ------------------------------------------------------------- multigrid
+def test_coarsen_and_face_masks_closed_form():
+    H, W = 8, 12
+    solid = np.zeros((H, W), dtype=bool)
+    solid[0, :] = solid[-1, :] = solid[:, 0] = solid[:, -1] = True
+    solid[3, 5] = True
+    sc = O.coarsen_solid_mask(solid)
+    exp = np.zeros((4, 6), dtype=bool)
+    exp[0, :] = exp[-1, :] = exp[:, 0] = exp[:, -1] = True
+    exp[1, 2] = True  # (3,5) lies in block (1,2)
+    np.testing.assert_array_equal(sc, exp)
+    fmu, fmv = O.build_face_masks(exp)
+    assert not fmu[:, 0].any() and not fmu[:, -1].any()
+    assert not fmv[0, :].any() and not fmv[-1, :].any()
+    assert fmu[2, 2] == (not (exp[2, 1] or exp[2, 2]))
+    assert fmv[2, 3] == (not (exp[1, 3] or exp[2, 3]))
+
+
+def test_restrict_closed_forms():
+    H, W = 6, 8
+    fine = np.full((H, W), 3.5)
+    fl = np.ones((H, W), dtype=bool)
+    np.testing.assert_array_equal(O.restrict(fine, fl), np.full((3, 4), 3.5))
+    fl[0:2, 0:2] = [[True, False], [False, False]]  # one fluid child
+    fl[0:2, 2:4] = False  # no fluid child
+    fine2 = np.arange(H * W, dtype=np.float64).reshape(H, W)
+    c = O.restrict(fine2, fl)
+    assert c[0, 0] == fine2[0, 0]
+    assert c[0, 1] == 0.0
+    assert c[1, 1] == (fine2[2, 2] + fine2[2, 3] + fine2[3, 2] + fine2[3, 3]) / 4
+
+
+def test_prolong_is_exact_on_linear_fields():
+    Hc, Wc = 6, 7
+    J, I = np.mgrid[0:Hc, 0:Wc].astype(np.float64)
+    coarse = 1.5 + 0.25 * I - 0.75 * J
+    fl = np.ones((2 * Hc, 2 * Wc), dtype=bool)
+    fine = O.prolong(coarse, fl)
+    j, i = np.mgrid[0:2 * Hc, 0:2 * Wc].astype(np.float64)
+    x = (i + 0.5) * 0.5 - 0.5
+    y = (j + 0.5) * 0.5 - 0.5
+    exp = 1.5 + 0.25 * x - 0.75 * y
+    inner = (x >= 0) & (x <= Wc - 1) & (y >= 0) & (y <= Hc - 1)
+    close(fine[inner], exp[inner])
+    # clamp-to-edge outside, zero on non-fluid fine cells
+    const = O.prolong(np.full((Hc, Wc), 2.0), fl)
+    np.testing.assert_array_equal(const, 2.0)
+    fl[3, 4] = False
+    assert O.prolong(coarse, fl)[3, 4] == 0.0
+
+
+def _dense_level(fmu, fmv):
+    H, W = fmu.shape[0], fmu.shape[1] - 1
+    fl = (fmu[:, :-1] | fmu[:, 1:] | fmv[:-1, :] | fmv[1:, :]).ravel()
+    return dict(H=H, W=W, N=adjacency(fmu, fmv), fl=fl, fmu=fmu, fmv=fmv)
+
+
+def _dense_rb(L, p, rhs, alpha, omega, sweeps):
+    H, W = L["H"], L["W"]
+    jj, ii = np.mgrid[0:H, 0:W]
+    colour = ((ii + jj) % 2).ravel()
+    b = alpha * rhs
+    for _ in range(sweeps):
+        for c in (0, 1):
+            sel = L["fl"] & (colour == c)
+            p = np.where(sel, (1 - omega) * p + omega * (L["N"] @ p + b) / 4.0, p)
+    return p
+
+
+def _restriction_matrix(L):
+    H, W = L["H"], L["W"]
+    R = np.zeros(((H // 2) * (W // 2), H * W))
+    for J in range(H // 2):
+        for I in range(W // 2):
+            ch = [(2 * J + a) * W + 2 * I + c for a in (0, 1) for c in (0, 1)]
+            ch = [k for k in ch if L["fl"][k]]
+            for k in ch:
+                R[J * (W // 2) + I, k] = 1.0 / len(ch)
+    return R
+
+
+def _prolongation_matrix(L, Hc, Wc):
+    """Bilinear weights of each fine cell centre on the coarse grid."""
+    P = np.zeros((L["H"] * L["W"], Hc * Wc))
+    e = np.zeros((Hc, Wc))
+    for q in range(Hc * Wc):  # column q = prolongation of the q-th unit vector
+        e.ravel()[:] = 0.0
+        e.ravel()[q] = 1.0
+        P[:, q] = O.prolong(e, L["fl"].reshape(L["H"], L["W"])).ravel()
+    return P
+
+
+@pytest.mark.parametrize("kind", ["open", "obstacles"])
+@pytest.mark.parametrize("scale", ["reference", "rediscretized"])
+def test_two_level_vcycle_is_dense_composition(kind, scale):
+    """One V-cycle on 2 levels = pre-smooth, r = b - A p, coarse rhs =
+    R r / alpha_c (reference) or R r / alpha (rediscretised), coarse smoothing
+    from zero, p += P e, post-smooth (solver.py:94-139)."""
+    H, W = 8, 10
+    solid, fmu, fmv, _ = masks(H, W, kind, seed=3)
+    L0 = _dense_level(fmu, fmv)
+    sc = O.coarsen_solid_mask(~L0["fl"].reshape(H, W))
+    cfmu, cfmv = O.build_face_masks(sc)
+    L1 = _dense_level(cfmu, cfmv)
+    rng = np.random.default_rng(9)
+    div = rng.standard_normal((H, W))
+    p0 = rng.standard_normal((H, W))
+    h, rho, dt, om = 0.25, 1.3, 0.02, 1.0
+    a0, a1 = O.calculate_alpha(h, rho, dt), O.calculate_alpha(2 * h, rho, dt)
+    rhs0 = div.ravel()
+    p = _dense_rb(L0, p0.ravel(), rhs0, a0, om, 2)
+    A0 = 4 * np.eye(H * W) - L0["N"]
+    r = np.where(L0["fl"], a0 * rhs0 - A0 @ p, 0.0)
+    rc = _restriction_matrix(L0) @ r / (a1 if scale == "reference" else a0)
+    e = _dense_rb(L1, np.zeros(rc.size), rc, a1, om, 50)
+    p = p + _prolongation_matrix(L0, H // 2, W // 2) @ e
+    p = _dense_rb(L0, p, rhs0, a0, om, 2)
+    out = p0.copy()
+    O.multigrid_solver(out, div, fmu, fmv, rho, h, dt, v_cycles=1,
+                       smoother_type="rb_sor_gauss_seidel", num_levels=2, smoother_iterations=2,
+                       coarsest_iterations=50, smoother_omega=om, coarse_scale=scale)
+    close(out.ravel(), p, 1e-11)
+
+
+def test_reference_coarse_correction_is_quarter_scaled():
+    """The claim behind coarse_scale: with an exact coarse solve the reference
+    scaling gives e_c = A_c^+ R r, a quarter of the rediscretised
+    A_c^+ (4 R r).  Checked on a two-level cycle without smoothing on the
+    fine level and a fully converged coarse level."""
+    H = W = 8
+    _, fmu, fmv, _ = masks(H, W, "open")
+    rng = np.random.default_rng(4)
+    u = rng.standard_normal((H, W + 1))
+    v = rng.standard_normal((H + 1, W))
+    div = np.zeros((H, W))
+    O.compute_divergence(u, v, div, fmu, fmv, 1.0)  # consistent rhs
+    kw = dict(v_cycles=1, smoother_type="rb_sor_gauss_seidel", num_levels=2,
+              smoother_iterations=0, coarsest_iterations=3000, smoother_omega=1.0)
+    p_ref, p_red = np.zeros((H, W)), np.zeros((H, W))
+    O.multigrid_solver(p_ref, div, fmu, fmv, 1.0, 1.0, 1.0, coarse_scale="reference", **kw)
+    O.multigrid_solver(p_red, div, fmu, fmv, 1.0, 1.0, 1.0, coarse_scale="rediscretized", **kw)
+    a, b = p_ref - p_ref.mean(), p_red - p_red.mean()
+    close(4.0 * a, b, 1e-9)
+
+
+# ---------------------------------------------------------- closed forms
+def test_forces_float32_scratch_and_row_rules():
+    H, W = 6, 5
+    rho = np.tile(np.arange(H, dtype=np.float64)[:, None] * 0.1, (1, W))
+    T = np.zeros((H, W))
+    v = np.zeros((H + 1, W))
+    dt, g, bc, dc, Ta = 0.016, -0.5, 5.0, 1.0, 0.0
+    O.apply_forces(v, rho, T, dt, g, bc, dc, Ta)
+    d = np.empty(H + 1)
+    d[0], d[H - 1], d[H] = rho[0, 0], rho[H - 1, 0], np.nan
+    for j in range(1, H - 1):
+        d[j] = 0.5 * (rho[j - 1, 0] + rho[j, 0])
+    for j in range(H):
+        bf = np.float32(-dc * d[j] * g)  # float32 scratch (forces.py:33)
+        tf = np.float32(-bc * (0.0 - Ta))
+        assert v[j, 0] == dt * float(np.float32(bf + tf)), j
+    assert v[H, 0] == dt * float(np.float32(0.0) + np.float32(-bc * (0.0 - Ta)))
+    # the float32 rounding is observable: 0.1 * 0.5 is not a float32
+    assert v[2, 0] != dt * (0.5 * (rho[1, 0] + rho[2, 0]) * 0.5)
+
+
+def test_temperature_force_rows():
+    H, W = 5, 3
+    T = np.tile(np.array([1.0, 2.0, 4.0, 8.0, 16.0])[:, None], (1, W))
+    v = np.zeros((H + 1, W))
+    O.apply_forces(v, np.zeros((H, W)), T, 1.0, -1.0, 2.0, 1.0, 1.0)
+    t = [1.0, 1.5, 3.0, 6.0, 12.0, 16.0]  # j=0 -> T[0], j=H -> T[H-1], else mean
+    for j in range(H + 1):
+        assert v[j, 1] == float(np.float32(0.0) + np.float32(-2.0 * (t[j] - 1.0)))
+
+
+def test_cooling_closed_form():
+    T = np.linspace(-2, 5, 24).reshape(4, 6)
+    out = T.copy()
+    O.cool_temperature(out, 0.5, 0.7, 0.016)
+    close(out, 0.5 + (T - 0.5) * math.exp(-0.7 * 0.016))
+
+
+def test_boundaries_closed_form():
+    H, W = 5, 6
+    solid = np.zeros((H, W), dtype=bool)
+    solid[2, 3] = True
+    u = np.ones((H, W + 1))
+    v = np.ones((H + 1, W))
+    O.enforce_boundary_conditions(u, v, solid)
+    zu = np.zeros((H, W + 1), dtype=bool)
+    zu[2, 3] = zu[2, 4] = True
+    zv = np.zeros((H + 1, W), dtype=bool)
+    zv[2, 3] = zv[3, 3] = True
+    np.testing.assert_array_equal(u == 0, zu)  # domain edges are not implied solid
+    np.testing.assert_array_equal(v == 0, zv)
+
+
+@pytest.mark.parametrize("approach,f", [
+    ("hard_cutoff", lambda q: 1.0),
+    ("linear", lambda q: 1.0 - q),
+    ("quadratic", lambda q: 1.0 - q * q),
+    ("gaussian", lambda q: math.exp(-4.5 * q * q)),
+])
+def test_emitter_falloffs(approach, f):
+    r = 6.0
+    for d in (0.0, 1.0, 2.5, 5.9):
+        assert abs(O.falloff(d, r, approach) - f(d / r)) <= 1e-15
+    assert O.falloff(r, r, approach) == 0.0
+    assert O.falloff(r + 1, r, approach) == 0.0
+    H, W = 20, 24
+    field = np.zeros((H, W))
+    O.emit_scalar(field, 2.0, 0.5, 10.0, 7.0, r, approach)
+    y, x = np.mgrid[0:H, 0:W]
+    dist = np.sqrt((x - 10.0) ** 2 + (y - 7.0) ** 2)  # cell indices, no +0.5
+    exp = np.where(dist < r, 2.0 * np.vectorize(lambda q: f(q))(np.minimum(dist / r, 1)) * 0.5, 0)
+    close(field, exp, 1e-14)
+
+
+def test_face_advection_bilinear_exactness():
+    """u / v face advection reproduces linear fields exactly (the reference
+    tests only scalar advection): with u = c0 + c1*i + c2*j on u faces and a
+    constant v, the backtraced bilinear sample is c0 + c1*x + c2*(y - 1/2)."""
+    H, W = 14, 16
+    c0, c1, c2, V, dt, h = 0.2, 0.03, -0.02, 0.4, 0.5, 1.0
+    ju, iu = np.mgrid[0:H, 0:W + 1].astype(np.float64)
+    u = c0 + c1 * iu + c2 * ju
+    v = np.full((H + 1, W), V)
+    fmu = np.ones((H, W + 1), dtype=bool)
+    out = np.zeros_like(u)
+    O.advect_velocity_u(u, out, v, fmu, dt, h)
+    x = iu - u * dt / h
+    y = ju + 0.5 - V * dt / h
+    exp = c0 + c1 * x + c2 * (y - 0.5)
+    inner = (x >= 0) & (x <= W) & (y - 0.5 >= 0) & (y - 0.5 <= H - 1)
+    assert inner.sum() > 100
+    close(out[inner], exp[inner], 1e-13)
+    # v faces: v = d0 + d1*i + d2*j, constant u
+    d0, d1, d2, U = -0.1, 0.02, 0.05, -0.3
+    jv, iv = np.mgrid[0:H + 1, 0:W].astype(np.float64)
+    vv = d0 + d1 * iv + d2 * jv
+    uu = np.full((H, W + 1), U)
+    outv = np.zeros_like(vv)
+    O.advect_velocity_v(vv, outv, uu, np.ones((H + 1, W), dtype=bool), dt, h)
+    x = iv + 0.5 - U * dt / h
+    y = jv - vv * dt / h
+    exp = d0 + d1 * (x - 0.5) + d2 * y
+    inner = (x - 0.5 >= 0) & (x - 0.5 <= W - 1) & (y >= 0) & (y <= H)
+    close(outv[inner], exp[inner], 1e-13)
+
+
+def test_advection_clamps_to_edge():
+    H, W = 8, 10
+    src = np.random.default_rng(2).random((H, W))
+    u = np.full((H, W + 1), 1e3)  # backtrace far to the left
+    v = np.zeros((H + 1, W))
+    dst = np.zeros((H, W))
+    O.advect_scalar_field(src, dst, u, v, np.ones((H, W), dtype=bool), 1.0, 1.0)
+    np.testing.assert_array_equal(dst, np.tile(src[:, :1], (1, W)))
