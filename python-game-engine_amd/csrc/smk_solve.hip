@@ -69,30 +69,40 @@ __device__ __forceinline__ double poisson_residual(T l, T r, T d, T u, T c, T rh
   return ((double)s + alpha * (double)rhs) - 4.0 * (double)c;
 }
 
-template <typename T, int MODE, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int R = a.R;
-  const int RW = TX + 2 * R, RH = TY + 2 * R, RN = RW * RH;
-  T* sp = reinterpret_cast<T*>(smem);
-  T* sr = sp + RN;
-  T* sq = (MODE == MODE_JACOBI) ? sr + RN : nullptr;  // Jacobi second buffer
-  uint8_t* sf = reinterpret_cast<uint8_t*>(sr + RN * (MODE == MODE_JACOBI ? 2 : 1));
+// Region = RH rows x 64 columns staged in LDS; lane x of every wavefront
+// owns region column x, the NW waves split the rows.  R is the halo, so the
+// output tile is (64 - 2R) x (RH - 2R).  Red-black passes walk row pairs:
+// in rows (2m, 2m+1) lane x updates the one cell of the pass's colour, so
+// every lane is busy and no index needs a division.
+template <typename T, int MODE, int RH, int NT>
+__global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
+  constexpr int RW = 64;
+  constexpr int NW = NT / 64;
+  __shared__ T sp[RH * RW];
+  __shared__ T sr[RH * RW];
+  __shared__ T sq[MODE == MODE_JACOBI ? RH * RW : 1];
+  __shared__ uint8_t sf[RH * RW];
+  __shared__ double red[NW];
 
+  const int R = a.R;
+  const int TX = RW - 2 * R, TY = RH - 2 * R;
   const int H = a.H, W = a.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int gx0 = blockIdx.x * TX - R, gy0 = blockIdx.y * TY - R;
+  const int gx = gx0 + lane;
+  const bool colin = gx >= 0 && gx < W;
 
   // ---- stage p (+ prolonged correction), rhs, flags
-  for (int k = threadIdx.x; k < RN; k += NT) {
-    const int ly = k / RW, lx = k - ly * RW;
-    const int gx = gx0 + lx, gy = gy0 + ly;
+  for (int ly = wave; ly < RH; ly += NW) {
+    const int gy = gy0 + ly;
+    const int o = ly * RW + lane;
     T pv = (T)0, rv = (T)0;
     uint8_t fv = 0;
-    if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-      const size_t o = (size_t)gy * W + gx;
-      fv = a.flags[o];
-      rv = a.rhs[o];
-      pv = a.zero_init ? (T)0 : a.p_in[o];
+    if (colin && gy >= 0 && gy < H) {
+      const size_t g = (size_t)gy * W + gx;
+      fv = a.flags[g];
+      rv = a.rhs[g];
+      pv = a.zero_init ? (T)0 : a.p_in[g];
       if (a.coarse_e != nullptr) {
         T corr = (T)0;
         if (fv & F_FLUID) {
@@ -102,9 +112,9 @@ __global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
         pv = pv + corr;
       }
     }
-    sp[k] = pv;
-    sr[k] = rv;
-    sf[k] = fv;
+    sp[o] = pv;
+    sr[o] = rv;
+    sf[o] = fv;
   }
   __syncthreads();
 
@@ -112,17 +122,14 @@ __global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
   T* cur = sp;
   for (int pass = 0; pass < a.npass; ++pass) {
     const int lo = pass + 1;
-    const int nrow = RH - 2 * lo, ncol = RW - 2 * lo;
-    if (nrow <= 0 || ncol <= 0) break;
+    const bool xin = lane >= lo && lane < RW - lo;
     if (MODE == MODE_RB) {
       const int colour = (a.first_colour + pass) & 1;
-      const int nh = (ncol + 1) >> 1;
-      for (int k = threadIdx.x; k < nrow * nh; k += NT) {
-        const int r = k / nh, c = k - r * nh;
-        const int ly = lo + r;
-        const int lx = lo + ((colour + ly + lo) & 1) + 2 * c;
-        if (lx >= RW - lo) continue;
-        const int o = ly * RW + lx;
+      const int par = (lane + colour) & 1;  // region origin has even parity
+      for (int pr = wave; pr < RH / 2; pr += NW) {
+        const int ly = 2 * pr + par;
+        if (!xin || ly < lo || ly >= RH - lo) continue;
+        const int o = ly * RW + lane;
         const uint8_t f = sf[o];
         if (!(f & F_FLUID)) continue;
         const T ce = cur[o];
@@ -135,9 +142,9 @@ __global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
       }
     } else {
       T* nxt = (cur == sp) ? sq : sp;
-      for (int k = threadIdx.x; k < nrow * ncol; k += NT) {
-        const int r = k / ncol, c = k - r * ncol;
-        const int o = (lo + r) * RW + lo + c;
+      for (int ly = wave; ly < RH; ly += NW) {
+        if (!xin || ly < lo || ly >= RH - lo) continue;
+        const int o = ly * RW + lane;
         const uint8_t f = sf[o];
         const T ce = cur[o];
         if (!(f & F_FLUID)) { nxt[o] = ce; continue; }
@@ -153,18 +160,21 @@ __global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
   }
 
   // ---- write the tile interior
-  for (int k = threadIdx.x; k < TX * TY; k += NT) {
-    const int ty = k / TX, tx = k - ty * TX;
-    const int gx = gx0 + R + tx, gy = gy0 + R + ty;
-    if (gx < W && gy < H) a.p_out[(size_t)gy * W + gx] = cur[(R + ty) * RW + R + tx];
+  const bool xout = lane >= R && lane < R + TX && colin;
+  for (int ly = R + wave; ly < R + TY; ly += NW) {
+    const int gy = gy0 + ly;
+    if (xout && gy < H) a.p_out[(size_t)gy * W + gx] = cur[ly * RW + lane];
   }
 
   // ---- fused residual -> restriction into the coarse rhs
   if (a.coarse_rhs != nullptr) {
+    const int CX = TX / 2, CY = TY / 2;
     const int Wc = W / 2, Hc = H / 2;
-    for (int k = threadIdx.x; k < (TX / 2) * (TY / 2); k += NT) {
-      const int cy = k / (TX / 2), cx = k - cy * (TX / 2);
-      const int Jc = blockIdx.y * (TY / 2) + cy, Ic = blockIdx.x * (TX / 2) + cx;
+    const int cx = lane & 31;
+    for (int cyb = 2 * wave; cyb < CY; cyb += 2 * NW) {
+      const int cy = cyb + (lane >> 5);
+      if (cy >= CY || cx >= CX) continue;
+      const int Jc = blockIdx.y * CY + cy, Ic = blockIdx.x * CX + cx;
       if (Jc >= Hc || Ic >= Wc) continue;
       double total = 0.0;
       int count = 0;
@@ -180,8 +190,8 @@ __global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
           const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
           const T d = (f & F_DOWN) ? cur[o - RW] : ce;
           const T u = (f & F_UP) ? cur[o + RW] : ce;
-          // residual stored in the level's dtype (solver.py:116-117), then
-          // summed in float64 (transfer.py:26-33)
+          // the residual is stored in the level's dtype (solver.py:116-117),
+          // then summed in float64 (transfer.py:26-33)
           const T rv = (T)poisson_residual(l, rr, d, u, ce, sr[o], a.alpha);
           total += (double)rv;
           count += 1;
@@ -191,15 +201,13 @@ __global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
     }
   }
 
-  // ---- fused residual norm (fluid interior cells)
+  // ---- fused residual norm over the fluid interior cells
   if (a.norm_partial != nullptr) {
-    __shared__ double red[NT / 64];
     double acc = 0.0;
-    for (int k = threadIdx.x; k < TX * TY; k += NT) {
-      const int ty = k / TX, tx = k - ty * TX;
-      const int gx = gx0 + R + tx, gy = gy0 + R + ty;
-      if (gx >= W || gy >= H) continue;
-      const int o = (R + ty) * RW + R + tx;
+    for (int ly = R + wave; ly < R + TY; ly += NW) {
+      const int gy = gy0 + ly;
+      if (!xout || gy >= H) continue;
+      const int o = ly * RW + lane;
       const uint8_t f = sf[o];
       if (!(f & F_FLUID)) continue;
       const T ce = cur[o];
@@ -215,28 +223,31 @@ __global__ __launch_bounds__(NT) void k_smooth_tiled(const SmoothArgs<T> a) {
   }
 }
 
-// Whole level in one workgroup's LDS: the coarsest solve (solver.py:107-110,
-// coarsest_iterations sweeps) without a launch per sweep.
+// The small levels of a V-cycle (top .. coarsest) in one workgroup with every
+// level's p, rhs and flags resident in LDS: pre-smooth and residual-restrict
+// going down, the coarsest smoothing, prolong-correct and post-smooth coming
+// up (solver.py:94-139), one launch instead of two per level.
+constexpr int SMALL_MAX = 8;
+
+template <typename T>
+struct SmallArgs {
+  int nl;
+  int H[SMALL_MAX], W[SMALL_MAX];
+  const uint8_t* flags[SMALL_MAX];
+  double alpha[SMALL_MAX], omega[SMALL_MAX], coarse_div[SMALL_MAX];
+  const T* rhs_top;
+  const T* p_top_in;
+  T* p_top_out;
+  int zero_top;
+  int npass_smooth;    // per pre/post smoothing (RB: 2*sweeps)
+  int npass_coarsest;  // coarsest level
+};
+
 template <typename T, int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_smooth_single(const T* __restrict__ p_in,
-                                                      T* __restrict__ p_out,
-                                                      const T* __restrict__ rhs,
-                                                      const uint8_t* __restrict__ flags,
-                                                      int H, int W, double alpha, double omega,
-                                                      int npass, int zero_init) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ void small_smooth(T* p, T* tmp, const T* rhs, const uint8_t* fl, int H, int W,
+                             double alpha, double omega, int npass) {
   const int N = H * W;
-  T* sp = reinterpret_cast<T*>(smem);
-  T* sr = sp + N;
-  T* sq = (MODE == MODE_JACOBI) ? sr + N : nullptr;
-  uint8_t* sf = reinterpret_cast<uint8_t*>(sr + N * (MODE == MODE_JACOBI ? 2 : 1));
-  for (int k = threadIdx.x; k < N; k += NT) {
-    sp[k] = zero_init ? (T)0 : p_in[k];
-    sr[k] = rhs[k];
-    sf[k] = flags[k];
-  }
-  __syncthreads();
-  T* cur = sp;
+  T* cur = p;
   for (int pass = 0; pass < npass; ++pass) {
     if (MODE == MODE_RB) {
       const int colour = pass & 1;
@@ -246,33 +257,128 @@ __global__ __launch_bounds__(NT) void k_smooth_single(const T* __restrict__ p_in
         const int i = ((colour + j) & 1) + 2 * c;
         if (i >= W) continue;
         const int o = j * W + i;
-        const uint8_t f = sf[o];
+        const uint8_t f = fl[o];
         if (!(f & F_FLUID)) continue;
         const T ce = cur[o];
         const T l = (f & F_LEFT) ? cur[o - 1] : ce;
         const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
         const T d = (f & F_DOWN) ? cur[o - W] : ce;
         const T u = (f & F_UP) ? cur[o + W] : ce;
-        const double nw = poisson_new(l, rr, d, u, sr[o], alpha);
+        const double nw = poisson_new(l, rr, d, u, rhs[o], alpha);
         cur[o] = (T)((1.0 - omega) * (double)ce + omega * nw);
       }
     } else {
-      T* nxt = (cur == sp) ? sq : sp;
+      T* nxt = (cur == p) ? tmp : p;
       for (int o = threadIdx.x; o < N; o += NT) {
-        const uint8_t f = sf[o];
+        const uint8_t f = fl[o];
         const T ce = cur[o];
         if (!(f & F_FLUID)) { nxt[o] = ce; continue; }
         const T l = (f & F_LEFT) ? cur[o - 1] : ce;
         const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
         const T d = (f & F_DOWN) ? cur[o - W] : ce;
         const T u = (f & F_UP) ? cur[o + W] : ce;
-        nxt[o] = (T)poisson_new(l, rr, d, u, sr[o], alpha);
+        nxt[o] = (T)poisson_new(l, rr, d, u, rhs[o], alpha);
       }
       cur = nxt;
     }
     __syncthreads();
   }
-  for (int k = threadIdx.x; k < N; k += NT) p_out[k] = cur[k];
+  if (cur != p) {  // Jacobi with an odd count: copy back (jacobi.py:51-52)
+    for (int o = threadIdx.x; o < N; o += NT) p[o] = cur[o];
+    __syncthreads();
+  }
+}
+
+template <typename T, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* P[SMALL_MAX];
+  T* B[SMALL_MAX];
+  uint8_t* F[SMALL_MAX];
+  unsigned char* cursor = smem;
+  int nmax = 0;
+  for (int l = 0; l < a.nl; ++l) {
+    const int n = a.H[l] * a.W[l];
+    nmax = n > nmax ? n : nmax;
+    P[l] = reinterpret_cast<T*>(cursor);
+    cursor += ((size_t)n * sizeof(T) + 15) & ~(size_t)15;
+    B[l] = reinterpret_cast<T*>(cursor);
+    cursor += ((size_t)n * sizeof(T) + 15) & ~(size_t)15;
+    F[l] = cursor;
+    cursor += ((size_t)n + 15) & ~(size_t)15;
+  }
+  T* tmp = reinterpret_cast<T*>(cursor);  // Jacobi ping-pong partner
+  (void)nmax;
+
+  // stage the top level
+  {
+    const int n = a.H[0] * a.W[0];
+    for (int o = threadIdx.x; o < n; o += NT) {
+      P[0][o] = a.zero_top ? (T)0 : a.p_top_in[o];
+      B[0][o] = a.rhs_top[o];
+      F[0][o] = a.flags[0][o];
+    }
+    for (int l = 1; l < a.nl; ++l) {
+      const int nl_ = a.H[l] * a.W[l];
+      for (int o = threadIdx.x; o < nl_; o += NT) {
+        F[l][o] = a.flags[l][o];
+        P[l][o] = (T)0;  // solver.py:127
+      }
+    }
+  }
+  __syncthreads();
+
+  const int last = a.nl - 1;
+  // down
+  for (int l = 0; l < last; ++l) {
+    const int H = a.H[l], W = a.W[l];
+    small_smooth<T, MODE, NT>(P[l], tmp, B[l], F[l], H, W, a.alpha[l], a.omega[l],
+                              a.npass_smooth);
+    const int Hc = a.H[l + 1], Wc = a.W[l + 1];
+    for (int k = threadIdx.x; k < Hc * Wc; k += NT) {
+      const int J = k / Wc, I = k - J * Wc;
+      double total = 0.0;
+      int count = 0;
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int o = (2 * J + dj) * W + 2 * I + di;
+          const uint8_t f = F[l][o];
+          if (!(f & F_FLUID)) continue;
+          const T* p = P[l];
+          const T ce = p[o];
+          const T lf = (f & F_LEFT) ? p[o - 1] : ce;
+          const T rt = (f & F_RIGHT) ? p[o + 1] : ce;
+          const T dn = (f & F_DOWN) ? p[o - W] : ce;
+          const T up = (f & F_UP) ? p[o + W] : ce;
+          const T rv = (T)poisson_residual(lf, rt, dn, up, ce, B[l][o], a.alpha[l]);
+          total += (double)rv;
+          count += 1;
+        }
+      const T mean = count > 0 ? (T)(total / (double)count) : (T)0;
+      B[l + 1][k] = (T)((double)mean / a.coarse_div[l]);
+    }
+    __syncthreads();
+  }
+  // coarsest
+  small_smooth<T, MODE, NT>(P[last], tmp, B[last], F[last], a.H[last], a.W[last],
+                            a.alpha[last], a.omega[last], a.npass_coarsest);
+  // up
+  for (int l = last - 1; l >= 0; --l) {
+    const int H = a.H[l], W = a.W[l];
+    const int Hc = a.H[l + 1], Wc = a.W[l + 1];
+    for (int o = threadIdx.x; o < H * W; o += NT) {
+      const int j = o / W, i = o - j * W;
+      T corr = (T)0;
+      if (F[l][o] & F_FLUID)
+        corr = (T)sample_field(P[l + 1], Hc, Wc, (i + 0.5) * 0.5 - 0.5, (j + 0.5) * 0.5 - 0.5);
+      P[l][o] = P[l][o] + corr;
+    }
+    __syncthreads();
+    small_smooth<T, MODE, NT>(P[l], tmp, B[l], F[l], H, W, a.alpha[l], a.omega[l],
+                              a.npass_smooth);
+  }
+  const int n0 = a.H[0] * a.W[0];
+  for (int o = threadIdx.x; o < n0; o += NT) a.p_top_out[o] = P[0][o];
 }
 
 constexpr int NB_X = 64, NB_Y = 4, NB_T = NB_X * NB_Y;
@@ -354,45 +460,86 @@ __global__ void k_solid_from_flags(const uint8_t* __restrict__ flags, long n,
 }
 
 // ------------------------------------------------------------------ host side
-constexpr int TILE_X = 32, TILE_Y = 32, TILE_NT = 256;
+constexpr int TILE_NT = 256;
+constexpr int TILE_RH = 48;
 constexpr int MAX_R = 8;
-constexpr size_t SINGLE_LDS_LIMIT = 64 * 1024;
+constexpr int SMALL_NT = 256;
 
-template <typename T>
-size_t tiled_lds_bytes(int mode, int R) {
-  const size_t rn = (size_t)(TILE_X + 2 * R) * (TILE_Y + 2 * R);
-  return rn * (sizeof(T) * (mode == MODE_JACOBI ? 3 : 2) + 1);
-}
-
-template <typename T>
-size_t single_lds_bytes(int mode, int H, int W) {
-  return (size_t)H * W * (sizeof(T) * (mode == MODE_JACOBI ? 3 : 2) + 1);
-}
+inline int tile_tx(int R) { return 64 - 2 * R; }
+inline int tile_ty(int R) { return TILE_RH - 2 * R; }
 
 template <typename T>
 int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s) {
-  dim3 grid(cdiv(a.W, TILE_X), cdiv(a.H, TILE_Y));
-  const size_t lds = tiled_lds_bytes<T>(mode, a.R);
+  if (a.R < 1 || a.R > MAX_R) return fail(SMK_EINVAL, "halo %d outside [1, %d]", a.R, MAX_R);
+  dim3 grid(cdiv(a.W, tile_tx(a.R)), cdiv(a.H, tile_ty(a.R)));
   if (mode == MODE_RB)
-    hipLaunchKernelGGL((k_smooth_tiled<T, MODE_RB, TILE_X, TILE_Y, TILE_NT>), grid,
-                       dim3(TILE_NT), lds, s, a);
+    hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, TILE_RH, TILE_NT>), grid, dim3(TILE_NT), 0,
+                       s, a);
   else
-    hipLaunchKernelGGL((k_smooth_tiled<T, MODE_JACOBI, TILE_X, TILE_Y, TILE_NT>), grid,
-                       dim3(TILE_NT), lds, s, a);
-  return check_launch("k_smooth_tiled");
+    hipLaunchKernelGGL((k_smooth_tile64<T, MODE_JACOBI, 32, TILE_NT>),
+                       dim3(cdiv(a.W, tile_tx(a.R)), cdiv(a.H, 32 - 2 * a.R)), dim3(TILE_NT), 0,
+                       s, a);
+  return check_launch("k_smooth_tile64");
+}
+
+// partial-sum slots one tiled launch can produce
+inline int tiled_blocks(int mode, int H, int W, int R) {
+  const int ty = mode == MODE_RB ? tile_ty(R) : 32 - 2 * R;
+  return (int)(cdiv(W, tile_tx(R)) * cdiv(H, ty));
+}
+
+inline size_t small_lds_bytes(const int* Hs, const int* Ws, int nl, size_t es, int mode) {
+  size_t b = 0, nmax = 0;
+  for (int l = 0; l < nl; ++l) {
+    const size_t n = (size_t)Hs[l] * Ws[l];
+    nmax = n > nmax ? n : nmax;
+    b += 2 * ((n * es + 15) & ~(size_t)15) + ((n + 15) & ~(size_t)15);
+  }
+  if (mode == MODE_JACOBI) b += nmax * es;
+  return b;
+}
+
+static size_t g_lds_limit = 0;
+
+// Largest dynamic LDS a workgroup may use on this device (opt-in limit when
+// the runtime reports one), and raise the kernels' attribute to it.
+template <typename T>
+static size_t small_lds_limit() {
+  static size_t lim = 0;
+  if (lim) return lim;
+  int dev = 0, a = 0, b = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+  size_t want = (size_t)(a > b ? a : b);
+  if (want > 160 * 1024) want = 160 * 1024;
+  if (want < 64 * 1024) want = 64 * 1024;
+  if (want > 64 * 1024) {
+    hipError_t e1 = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, SMALL_NT>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)want);
+    hipError_t e2 = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, SMALL_NT>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)want);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      (void)hipGetLastError();
+      want = 64 * 1024;
+    }
+  }
+  lim = want;
+  g_lds_limit = want;
+  return lim;
 }
 
 template <typename T>
-int launch_single(int mode, const T* pin, T* pout, const T* rhs, const uint8_t* flags, int H,
-                  int W, double alpha, double omega, int npass, int zero_init, hipStream_t s) {
-  const size_t lds = single_lds_bytes<T>(mode, H, W);
+int launch_small(int mode, const SmallArgs<T>& a, size_t lds, hipStream_t s) {
   if (mode == MODE_RB)
-    hipLaunchKernelGGL((k_smooth_single<T, MODE_RB, 1024>), dim3(1), dim3(1024), lds, s, pin,
-                       pout, rhs, flags, H, W, alpha, omega, npass, zero_init);
+    hipLaunchKernelGGL((k_vcycle_small<T, MODE_RB, SMALL_NT>), dim3(1), dim3(SMALL_NT), lds, s,
+                       a);
   else
-    hipLaunchKernelGGL((k_smooth_single<T, MODE_JACOBI, 1024>), dim3(1), dim3(1024), lds, s,
-                       pin, pout, rhs, flags, H, W, alpha, omega, npass, zero_init);
-  return check_launch("k_smooth_single");
+    hipLaunchKernelGGL((k_vcycle_small<T, MODE_JACOBI, SMALL_NT>), dim3(1), dim3(SMALL_NT), lds,
+                       s, a);
+  return check_launch("k_vcycle_small");
 }
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -615,6 +762,8 @@ struct smk_mg {
   double* partial;
   double* result;
   int npartial;
+  int small_from[2];  // first level handled by k_vcycle_small: [native dtype, float32]
+  int small_mode[2];  // smoother mode small_from was computed for
 };
 
 namespace smk {
@@ -633,25 +782,41 @@ struct MgRun {
   double rho, dt;
 };
 
-// Smooth level `l` with `npass` passes from buffer index `src` (or zero) into
-// the other buffer chain, optionally prolonging a coarse correction in the
-// first launch and restricting the residual in the last.  Updates lv.cur.
+inline double level_omega(const MgRun& run, const Level& L) {
+  if (run.mode != MODE_RB) return 1.0;
+  return run.omega_override > 0.0 ? run.omega_override : optimal_omega(L.H > L.W ? L.H : L.W);
+}
+
+// First level from which the rest of the hierarchy fits one workgroup's LDS.
 template <typename T>
-int mg_smooth(smk_mg* mg, std::vector<Level>& lv, int l, bool use32, int npass, bool zero_init,
+int mg_small_from(smk_mg* mg, int mode) {
+  const size_t lim = small_lds_limit<T>();
+  int Hs[64], Ws[64];
+  for (int l = 0; l < mg->nl; ++l) {
+    Hs[l] = mg->lv[l].H;
+    Ws[l] = mg->lv[l].W;
+  }
+  for (int from = 0; from < mg->nl; ++from) {
+    if (mg->nl - from > SMALL_MAX) continue;
+    if (small_lds_bytes(Hs + from, Ws + from, mg->nl - from, sizeof(T), mode) <= lim) return from;
+  }
+  return -1;  // the coarsest level alone does not fit: tiled kernels only
+}
+
+// Smooth level l with npass passes (tiled kernels), optionally prolonging a
+// coarse correction in the first launch and restricting the residual or
+// reducing the residual norm in the last.  Flips lv.cur per launch.
+template <typename T>
+int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_init,
               const T* coarse_e, int Hc, int Wc, T* coarse_rhs, double coarse_div,
-              double alpha, double omega, int mode, hipStream_t s) {
+              double* norm_partial, int* nparts, double alpha, double omega, int mode,
+              hipStream_t s) {
   Level& L = lv[l];
   void** pb = use32 ? L.p32 : L.p;
   int& cur = use32 ? L.cur32 : L.cur;
   const T* rhs = reinterpret_cast<const T*>(use32 ? L.rhs32 : L.rhs);
-  const bool restrict_out = coarse_rhs != nullptr;
-  const int extra = restrict_out ? 1 : 0;
-  // coarsest / small level: the whole level in one workgroup
-  if (!restrict_out && coarse_e == nullptr && single_lds_bytes<T>(mode, L.H, L.W) <= SINGLE_LDS_LIMIT) {
-    T* buf = reinterpret_cast<T*>(pb[cur]);
-    return launch_single<T>(mode, buf, buf, rhs, L.flags, L.H, L.W, alpha, omega, npass,
-                            zero_init ? 1 : 0, s);
-  }
+  const bool tail = coarse_rhs != nullptr || norm_partial != nullptr;
+  const int extra = tail ? 1 : 0;
   std::vector<int> chunks;
   for (int left = npass; left > 0;) {
     const int c = left < (MAX_R - extra) ? left : (MAX_R - extra);
@@ -673,7 +838,7 @@ int mg_smooth(smk_mg* mg, std::vector<Level>& lv, int l, bool use32, int npass, 
     a.omega = omega;
     a.npass = chunks[q];
     a.first_colour = colour;
-    a.R = chunks[q] + ((last && restrict_out) ? 1 : 0);
+    a.R = chunks[q] + ((last && tail) ? 1 : 0);
     if (a.R < 1) a.R = 1;
     a.zero_init = (first && zero_init) ? 1 : 0;
     if (first && coarse_e != nullptr) {
@@ -681,11 +846,14 @@ int mg_smooth(smk_mg* mg, std::vector<Level>& lv, int l, bool use32, int npass, 
       a.Hc = Hc;
       a.Wc = Wc;
     }
-    if (last && restrict_out) {
+    a.coarse_div = 1.0;
+    if (last && coarse_rhs != nullptr) {
       a.coarse_rhs = coarse_rhs;
       a.coarse_div = coarse_div;
-    } else {
-      a.coarse_div = 1.0;
+    }
+    if (last && norm_partial != nullptr) {
+      a.norm_partial = norm_partial;
+      *nparts = tiled_blocks(mode, L.H, L.W, a.R);
     }
     int rc = launch_tiled<T>(mode, a, s);
     if (rc) return rc;
@@ -695,33 +863,109 @@ int mg_smooth(smk_mg* mg, std::vector<Level>& lv, int l, bool use32, int npass, 
   return SMK_OK;
 }
 
-// solver.py:94-139
+// Levels l .. nl-1 in one k_vcycle_small launch (in place on p[cur]).
+template <typename T>
+int mg_small(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zero_init,
+             hipStream_t s) {
+  std::vector<Level>& lv = mg->lv;
+  SmallArgs<T> a{};
+  a.nl = mg->nl - l;
+  double hl = h;
+  for (int k = 0; k < a.nl; ++k) {
+    const Level& L = lv[l + k];
+    a.H[k] = L.H;
+    a.W[k] = L.W;
+    a.flags[k] = L.flags;
+    a.alpha[k] = calc_alpha(hl, run.rho, run.dt);
+    a.omega[k] = level_omega(run, L);
+    a.coarse_div[k] = run.rediscretize ? a.alpha[k] : calc_alpha(hl * 2.0, run.rho, run.dt);
+    hl *= 2.0;
+  }
+  Level& T0 = lv[l];
+  void** pb = use32 ? T0.p32 : T0.p;
+  const int cur = use32 ? T0.cur32 : T0.cur;
+  a.rhs_top = reinterpret_cast<const T*>(use32 ? T0.rhs32 : T0.rhs);
+  a.p_top_in = reinterpret_cast<const T*>(pb[cur]);
+  a.p_top_out = reinterpret_cast<T*>(pb[cur]);
+  a.zero_top = zero_init ? 1 : 0;
+  a.npass_smooth = run.mode == MODE_RB ? 2 * run.si : run.si;
+  a.npass_coarsest = run.mode == MODE_RB ? 2 * run.ci : run.ci;
+  const size_t lds = small_lds_bytes(a.H, a.W, a.nl, sizeof(T), run.mode);
+  return launch_small<T>(run.mode, a, lds, s);
+}
+
+// solver.py:94-139.  norm_partial (level 0 only): reduce ||r||^2 of the
+// result in the last post-smooth launch; *nparts receives the slot count.
 template <typename T>
 int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zero_init,
-              hipStream_t s) {
+              double* norm_partial, int* nparts, hipStream_t s) {
   std::vector<Level>& lv = mg->lv;
+  const int from = mg->small_from[use32 ? 1 : 0];
+  if (from >= 0 && l >= from) {
+    int rc = mg_small<T>(mg, l, use32, run, h, zero_init, s);
+    if (rc || norm_partial == nullptr) return rc;
+    *nparts = -1;  // caller computes the norm separately
+    return SMK_OK;
+  }
   Level& L = lv[l];
   const bool coarsest = (l == mg->nl - 1);
   const int n_iter = coarsest ? run.ci : run.si;
   const int npass = run.mode == MODE_RB ? 2 * n_iter : n_iter;
   const double alpha = calc_alpha(h, run.rho, run.dt);
-  double omega = 1.0;
-  if (run.mode == MODE_RB)
-    omega = run.omega_override > 0.0 ? run.omega_override : optimal_omega(L.H > L.W ? L.H : L.W);
+  const double omega = level_omega(run, L);
   if (coarsest)
-    return mg_smooth<T>(mg, lv, l, use32, npass, zero_init, nullptr, 0, 0, nullptr, 1.0, alpha,
-                        omega, run.mode, s);
+    return mg_smooth<T>(lv, l, use32, npass, zero_init, nullptr, 0, 0, nullptr, 1.0,
+                        norm_partial, nparts, alpha, omega, run.mode, s);
   Level& C = lv[l + 1];
   T* crhs = reinterpret_cast<T*>(use32 ? C.rhs32 : C.rhs);
   const double coarse_div = run.rediscretize ? alpha : calc_alpha(h * 2.0, run.rho, run.dt);
-  int rc = mg_smooth<T>(mg, lv, l, use32, npass, zero_init, nullptr, 0, 0, crhs, coarse_div,
-                        alpha, omega, run.mode, s);
+  int rc = mg_smooth<T>(lv, l, use32, npass, zero_init, nullptr, 0, 0, crhs, coarse_div, nullptr,
+                        nullptr, alpha, omega, run.mode, s);
   if (rc) return rc;
-  rc = mg_vcycle<T>(mg, l + 1, use32, run, h * 2.0, true, s);
+  rc = mg_vcycle<T>(mg, l + 1, use32, run, h * 2.0, true, nullptr, nullptr, s);
   if (rc) return rc;
   const T* ce = reinterpret_cast<const T*>(use32 ? C.p32[C.cur32] : C.p[C.cur]);
-  return mg_smooth<T>(mg, lv, l, use32, npass, false, ce, C.H, C.W, nullptr, 1.0, alpha, omega,
-                      run.mode, s);
+  return mg_smooth<T>(lv, l, use32, npass, false, ce, C.H, C.W, nullptr, 1.0, norm_partial,
+                      nparts, alpha, omega, run.mode, s);
+}
+
+inline MgRun make_run(const smk_mg_params* prm, double rho, double dt) {
+  MgRun run{};
+  run.mode = prm->smoother == SMK_SMOOTH_JACOBI ? MODE_JACOBI : MODE_RB;
+  run.omega_override = prm->smoother_omega;
+  run.si = prm->smoother_iterations;
+  run.ci = prm->coarsest_iterations;
+  run.rediscretize = prm->coarse_scale == SMK_COARSE_REDISCRETIZED;
+  run.rho = rho;
+  run.dt = dt;
+  return run;
+}
+
+template <typename T>
+void mg_prepare_small(smk_mg* mg, int slot, int mode) {
+  if (mg->small_mode[slot] != mode) {
+    mg->small_from[slot] = mg_small_from<T>(mg, mode);
+    mg->small_mode[slot] = mode;
+  }
+}
+
+template <typename T>
+int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, double* host,
+              int count, hipStream_t s) {
+  Level& L0 = mg->lv[0];
+  int rc;
+  if (nparts < 0) {
+    rc = residual_norm2_dev<T>(p, div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 0,
+                               nullptr, s);
+  } else {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
+                       mg->result, 0);
+    rc = check_launch("k_sum_partials");
+  }
+  if (rc) return rc;
+  SMK_CHECK_HIP(hipMemcpyAsync(host, mg->result, count * sizeof(double), hipMemcpyDeviceToHost, s));
+  SMK_CHECK_HIP(hipStreamSynchronize(s));
+  return SMK_OK;
 }
 
 template <typename T>
@@ -731,14 +975,8 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
   L0.p[0] = p;
   L0.rhs = const_cast<T*>(div);
   L0.cur = 0;
-  MgRun run{};
-  run.mode = prm->smoother == SMK_SMOOTH_JACOBI ? MODE_JACOBI : MODE_RB;
-  run.omega_override = prm->smoother_omega;
-  run.si = prm->smoother_iterations;
-  run.ci = prm->coarsest_iterations;
-  run.rediscretize = prm->coarse_scale == SMK_COARSE_REDISCRETIZED;
-  run.rho = rho;
-  run.dt = dt;
+  const MgRun run = make_run(prm, rho, dt);
+  mg_prepare_small<T>(mg, 0, run.mode);
   const double alpha = calc_alpha(h, rho, dt);
   int rc;
   int cycles = 0;
@@ -747,30 +985,25 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
   if (prm->tolerance > 0.0) {
     rc = rhs_norm2_dev<T>(div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 1, s);
     if (rc) return rc;
-    rc = residual_norm2_dev<T>(p, div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 0,
-                               nullptr, s);
-    if (rc) return rc;
     double host[2];
-    SMK_CHECK_HIP(hipMemcpyAsync(host, mg->result, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    SMK_CHECK_HIP(hipStreamSynchronize(s));
+    rc = read_norm<T>(mg, -1, p, div, alpha, host, 2, s);
+    if (rc) return rc;
     const double b2 = host[1];
     rel = b2 > 0.0 ? sqrt(host[0] / b2) : 0.0;
     while (rel > prm->tolerance && cycles < prm->v_cycles) {
-      rc = mg_vcycle<T>(mg, 0, false, run, h, false, s);
+      int nparts = -1;
+      rc = mg_vcycle<T>(mg, 0, false, run, h, false, mg->partial, &nparts, s);
       if (rc) return rc;
       cycles++;
       const T* pc = reinterpret_cast<const T*>(L0.p[L0.cur]);
-      rc = residual_norm2_dev<T>(pc, div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result,
-                                 0, nullptr, s);
+      rc = read_norm<T>(mg, nparts, pc, div, alpha, host, 1, s);
       if (rc) return rc;
-      SMK_CHECK_HIP(hipMemcpyAsync(host, mg->result, sizeof(double), hipMemcpyDeviceToHost, s));
-      SMK_CHECK_HIP(hipStreamSynchronize(s));
       rel = b2 > 0.0 ? sqrt(host[0] / b2) : 0.0;
     }
     converged = rel <= prm->tolerance;
   } else {
     for (int c = 0; c < prm->v_cycles; ++c) {
-      rc = mg_vcycle<T>(mg, 0, false, run, h, false, s);
+      rc = mg_vcycle<T>(mg, 0, false, run, h, false, nullptr, nullptr, s);
       if (rc) return rc;
       cycles++;
     }
@@ -812,14 +1045,8 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
   int rc = mg_ensure32(mg);
   if (rc) return rc;
   Level& L0 = mg->lv[0];
-  MgRun run{};
-  run.mode = prm->smoother == SMK_SMOOTH_JACOBI ? MODE_JACOBI : MODE_RB;
-  run.omega_override = prm->smoother_omega;
-  run.si = prm->smoother_iterations;
-  run.ci = prm->coarsest_iterations;
-  run.rediscretize = prm->coarse_scale == SMK_COARSE_REDISCRETIZED;
-  run.rho = rho;
-  run.dt = dt;
+  const MgRun run = make_run(prm, rho, dt);
+  mg_prepare_small<float>(mg, 1, run.mode);
   const double alpha = calc_alpha(h, rho, dt);
   const int H = mg->H, W = mg->W;
   const long n = (long)H * W;
@@ -842,7 +1069,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     if (tol > 0.0 ? rel <= tol : refinements >= prm->v_cycles) break;
     if (inner_total >= prm->v_cycles) break;
     for (int c = 0; c < inner; ++c) {
-      rc = mg_vcycle<float>(mg, 0, true, run, h, c == 0, s);
+      rc = mg_vcycle<float>(mg, 0, true, run, h, c == 0, nullptr, nullptr, s);
       if (rc) return rc;
       inner_total++;
     }
@@ -966,6 +1193,8 @@ int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W, int num_
   *out = nullptr;
   hipStream_t s = as_stream(stream);
   smk_mg* mg = new smk_mg();
+  mg->small_from[0] = mg->small_from[1] = -1;
+  mg->small_mode[0] = mg->small_mode[1] = -1;
   mg->H = H;
   mg->W = W;
   mg->dtype = dtype;
