@@ -54,7 +54,48 @@ def algorithmic_bytes_per_cell(v_cycles_per_step: float, mixed: bool, levels: in
     return 168.0 + 48.0 + v_cycles_per_step * (per_vc64 + 16.0) + 16.0
 
 
-def run_variant(scene, precision: str, steps: int, warmup: int):
+def profile_steps(sim, scene, n: int) -> list:
+    """Per kernel family over n extra (untimed) steps: launches, ms, algorithmic
+    bytes, recorded by the library with HIP events on the launch stream."""
+    import torch
+
+    from cfd_amd.backends.hip import _lib
+
+    torch.cuda.synchronize()
+    _lib.profile_read(reset=True)
+    _lib.profile_enable(True)
+    try:
+        for _ in range(n):
+            sim.substep(scene.dt)
+        recs = _lib.profile_read(reset=True)
+    finally:
+        _lib.profile_enable(False)
+    for r in recs:
+        r["avg_launch_us"] = 1e3 * r["total_ms"] / max(r["launches"], 1)
+        r["alg_bytes_per_launch"] = r["alg_bytes"] / max(r["launches"], 1)
+        r["alg_GBps"] = r["alg_bytes"] / max(r["total_ms"], 1e-12) / 1e6
+    return sorted(recs, key=lambda r: -r["total_ms"])
+
+
+def calibrate_copy(nbytes: int = 256 << 20) -> float:
+    """A plain device copy of known size (nbytes read + nbytes written), run
+    once so that HBM byte counters (FETCH_SIZE / WRITE_SIZE) of a profiled run
+    can be scaled; returns its GB/s."""
+    import torch
+
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2 * nbytes / (e0.elapsed_time(e1) * 1e6)
+
+
+def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3):
     import torch
 
     from cfd_amd import scenes
@@ -80,7 +121,9 @@ def run_variant(scene, precision: str, steps: int, warmup: int):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(end)
+    kernels = profile_steps(sim, scene, profile) if profile > 0 else []
     return {
+        "kernels": kernels,
         "ms_per_step": ms / steps,
         "wall_ms_per_step": wall * 1e3 / steps,
         "cell_steps_per_s": scene.cells * steps / (ms / 1e3),
@@ -149,6 +192,13 @@ def main():
 
     main_var = "mixed" if "mixed" in results else list(results)[0]
     m = results[main_var]
+    copy_gbps = calibrate_copy()
+    dom = m["kernels"][0] if m["kernels"] else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if dom is not None and os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dom["name"], {}).get("hbm_bytes_per_launch")
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_steps} oracle steps")
@@ -173,9 +223,13 @@ def main():
                              "rediscretized coarse scale",
                    "variant": main_var},
         "variants": results,
-        "roofline": {"bound": "hbm", "achieved": m["alg_GBps_whole_step"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": m["alg_GBps_whole_step"] / PEAK_HBM_GBS,
-                     "traffic": None, "scope": "whole step (provisional)"},
+        "roofline": None if dom is None else {
+            "bound": "hbm", "kernel": dom["name"], "achieved": dom["alg_GBps"],
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["alg_GBps"] / PEAK_HBM_GBS,
+            "traffic": traffic, "avg_launch_us": dom["avg_launch_us"],
+            "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
+            "whole_step_alg_GBps": m["alg_GBps_whole_step"],
+            "copy_kernel_GBps": copy_gbps},
         "cpu_baseline": cpu,
     }
     if "fp64" in results and "mixed" in results:

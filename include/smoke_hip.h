@@ -221,6 +221,28 @@ int smk_mg_solve(smk_mg* mg, void* p, const void* div, double fluid_density,
                  smk_solve_info* info, void* stream);
 int smk_mg_destroy(smk_mg* mg);
 
+/* ------------------------------------------------- profiling / tuning */
+/* No reference counterpart: measurement and tuning hooks.
+ * smk_profile_enable(1) makes every solver launch record a HIP event pair on
+ * its stream, tagged with the launch's algorithmic bytes; smk_profile_read
+ * synchronises, aggregates by kernel family (first-seen order) into up to n
+ * records, optionally clears, and returns the number of families. */
+typedef struct smk_profile_record {
+  char name[48];
+  int launches;
+  double total_ms;
+  double alg_bytes;
+} smk_profile_record;
+int smk_profile_enable(int on);
+int smk_profile_read(smk_profile_record* out, int n, int reset);
+
+/* Red-black smoother tile variant per dtype for the fine level (which = 0:
+ * finest multigrid level and the single-grid solvers) and the coarse levels
+ * (which = 1): 0 = 64x48 LDS tile, 1 = 64x32 LDS tile, 2 = 64x64
+ * register-resident region.  Results do not depend on the variant. */
+int smk_set_smoother_variant(int dtype, int which, int variant);
+int smk_get_smoother_variant(int dtype, int which);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,15 @@ class SolveInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ProfileRecord(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 48),
+        ("launches", ctypes.c_int),
+        ("total_ms", ctypes.c_double),
+        ("alg_bytes", ctypes.c_double),
+    ]
+
+
 class MgParams(ctypes.Structure):
     _fields_ = [
         ("v_cycles", ctypes.c_int),
@@ -90,6 +99,10 @@ SIGNATURES = {
     "smk_mg_solve": (_i, [_vp, _vp, _vp, _d, _d, _d, ctypes.POINTER(MgParams),
                           ctypes.POINTER(SolveInfo), _vp]),
     "smk_mg_destroy": (_i, [_vp]),
+    "smk_profile_enable": (_i, [_i]),
+    "smk_profile_read": (_i, [ctypes.POINTER(ProfileRecord), _i, _i]),
+    "smk_set_smoother_variant": (_i, [_i, _i, _i]),
+    "smk_get_smoother_variant": (_i, [_i, _i]),
 }
 
 _lib = None
@@ -137,3 +150,27 @@ def call(name: str, *args) -> int:
 
 def stream_of(t: torch.Tensor):
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def profile_enable(on: bool) -> None:
+    load().smk_profile_enable(1 if on else 0)
+
+
+def profile_read(reset: bool = True) -> list:
+    """Per kernel family: launches, total_ms, alg_bytes (synchronises)."""
+    L = load()
+    buf = (ProfileRecord * 64)()
+    n = L.smk_profile_read(buf, 64, 1 if reset else 0)
+    if n < 0:
+        raise HipError(L.smk_last_error().decode(errors="replace"))
+    return [{"name": buf[k].name.decode(), "launches": buf[k].launches,
+             "total_ms": buf[k].total_ms, "alg_bytes": buf[k].alg_bytes}
+            for k in range(min(n, 64))]
+
+
+SMOOTHER_VARIANTS = {"lds48": 0, "lds32": 1, "reg64": 2}
+
+
+def set_smoother_variant(dtype_code: int, which: str, variant: str) -> None:
+    call("smk_set_smoother_variant", int(dtype_code), 0 if which == "fine" else 1,
+         SMOOTHER_VARIANTS[variant])

@@ -69,13 +69,43 @@ __device__ __forceinline__ double poisson_residual(T l, T r, T d, T u, T c, T rh
   return ((double)s + alpha * (double)rhs) - 4.0 * (double)c;
 }
 
+
+// Branch-free stencil reads: the centre, the four neighbours and rhs are
+// loaded unconditionally (index o must keep o-1, o+1, o-S, o+S in range),
+// then the flags select which neighbour enters the sum, as the reference's
+// masked conditional reads (residual.py:35-38) do.
+template <typename T>
+__device__ __forceinline__ void stencil_load(const T* cur, const T* rhs, int o, int S,
+                                             uint8_t f, T& ce, T& nsum, T& b) {
+  ce = cur[o];
+  const T lv = cur[o - 1], rv = cur[o + 1], dv = cur[o - S], uv = cur[o + S];
+  b = rhs[o];
+  const T l = (f & F_LEFT) ? lv : ce;
+  const T r = (f & F_RIGHT) ? rv : ce;
+  const T d = (f & F_DOWN) ? dv : ce;
+  const T u = (f & F_UP) ? uv : ce;
+  nsum = l + r + d + u;
+}
+
+template <typename T>
+__device__ __forceinline__ double relax_value(int mode, T ce, T nsum, T b, double alpha,
+                                              double omega) {
+  const double nw = ((double)nsum + alpha * (double)b) * 0.25;
+  return mode == 0 ? (1.0 - omega) * (double)ce + omega * nw : nw;
+}
+
+template <typename T>
+__device__ __forceinline__ double residual_value(T ce, T nsum, T b, double alpha) {
+  return ((double)nsum + alpha * (double)b) - 4.0 * (double)ce;
+}
+
 // Region = RH rows x 64 columns staged in LDS; lane x of every wavefront
 // owns region column x, the NW waves split the rows.  R is the halo, so the
 // output tile is (64 - 2R) x (RH - 2R).  Red-black passes walk row pairs:
 // in rows (2m, 2m+1) lane x updates the one cell of the pass's colour, so
 // every lane is busy and no index needs a division.
 template <typename T, int MODE, int RH, int NT>
-__global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
+__device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a) {
   constexpr int RW = 64;
   constexpr int NW = NT / 64;
   __shared__ T sp[RH * RW];
@@ -92,29 +122,44 @@ __global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
   const int gx = gx0 + lane;
   const bool colin = gx >= 0 && gx < W;
 
-  // ---- stage p (+ prolonged correction), rhs, flags
-  for (int ly = wave; ly < RH; ly += NW) {
-    const int gy = gy0 + ly;
-    const int o = ly * RW + lane;
-    T pv = (T)0, rv = (T)0;
-    uint8_t fv = 0;
-    if (colin && gy >= 0 && gy < H) {
-      const size_t g = (size_t)gy * W + gx;
-      fv = a.flags[g];
-      rv = a.rhs[g];
-      pv = a.zero_init ? (T)0 : a.p_in[g];
-      if (a.coarse_e != nullptr) {
-        T corr = (T)0;
-        if (fv & F_FLUID) {
-          const double xc = (gx + 0.5) * 0.5, yc = (gy + 0.5) * 0.5;
-          corr = (T)sample_field(a.coarse_e, a.Hc, a.Wc, xc - 0.5, yc - 0.5);
-        }
-        pv = pv + corr;
-      }
+  // ---- stage p (+ prolonged correction), rhs, flags.  Addresses are
+  // clamped into the grid so every load is unconditional: all of a lane's
+  // loads are issued back to back and the lane waits once, instead of once
+  // per row behind a bounds branch.  Out-of-grid cells are then zeroed.
+  static_assert(RH % (2 * NW) == 0, "row pairs must split evenly over the waves");
+  constexpr int KR = RH / NW;
+  const int gxc = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+  T pv[KR], rv[KR];
+  uint8_t fv[KR];
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int gy = gy0 + wave + k * NW;
+    const int gyc = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+    const size_t g = (size_t)gyc * W + gxc;
+    fv[k] = a.flags[g];
+    rv[k] = a.rhs[g];
+    pv[k] = a.zero_init ? (T)0 : a.p_in[g];
+  }
+  if (a.coarse_e != nullptr) {
+    T corr[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      const int gy = gy0 + wave + k * NW;
+      const double xc = (gx + 0.5) * 0.5, yc = (gy + 0.5) * 0.5;
+      corr[k] = (T)sample_field(a.coarse_e, a.Hc, a.Wc, xc - 0.5, yc - 0.5);
     }
-    sp[o] = pv;
-    sr[o] = rv;
-    sf[o] = fv;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) pv[k] = pv[k] + ((fv[k] & F_FLUID) ? corr[k] : (T)0);
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int ly = wave + k * NW;
+    const int gy = gy0 + ly;
+    const bool in = colin && gy >= 0 && gy < H;
+    const int o = ly * RW + lane;
+    sp[o] = in ? pv[k] : (T)0;
+    sr[o] = in ? rv[k] : (T)0;
+    sf[o] = in ? fv[k] : (uint8_t)0;
   }
   __syncthreads();
 
@@ -126,34 +171,49 @@ __global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
     if (MODE == MODE_RB) {
       const int colour = (a.first_colour + pass) & 1;
       const int par = (lane + colour) & 1;  // region origin has even parity
-      for (int pr = wave; pr < RH / 2; pr += NW) {
-        const int ly = 2 * pr + par;
-        if (!xin || ly < lo || ly >= RH - lo) continue;
-        const int o = ly * RW + lane;
+      // Cells of one colour never neighbour each other: every new value of
+      // the lane is computed before any is stored, and cells outside the
+      // valid window read a safe interior index and are not stored.
+      constexpr int K = RH / (2 * NW);
+      T nv[K];
+      int oo[K];
+      bool st[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int ly = 2 * (wave + k * NW) + par;
+        const bool valid = xin && ly >= lo && ly < RH - lo;
+        const int o = valid ? ly * RW + lane : RW + 1;
         const uint8_t f = sf[o];
-        if (!(f & F_FLUID)) continue;
-        const T ce = cur[o];
-        const T l = (f & F_LEFT) ? cur[o - 1] : ce;
-        const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
-        const T d = (f & F_DOWN) ? cur[o - RW] : ce;
-        const T u = (f & F_UP) ? cur[o + RW] : ce;
-        const double nw = poisson_new(l, rr, d, u, sr[o], a.alpha);
-        cur[o] = (T)((1.0 - a.omega) * (double)ce + a.omega * nw);
+        T ce, ns, b;
+        stencil_load(cur, sr, o, RW, f, ce, ns, b);
+        nv[k] = (T)relax_value(0, ce, ns, b, a.alpha, a.omega);
+        oo[k] = o;
+        st[k] = valid && (f & F_FLUID);
       }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (st[k]) cur[oo[k]] = nv[k];
     } else {
       T* nxt = (cur == sp) ? sq : sp;
-      for (int ly = wave; ly < RH; ly += NW) {
-        if (!xin || ly < lo || ly >= RH - lo) continue;
-        const int o = ly * RW + lane;
+      constexpr int K = RH / NW;
+      T nv[K];
+      int oo[K];
+      bool st[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int ly = wave + k * NW;
+        const bool valid = xin && ly >= lo && ly < RH - lo;
+        const int o = valid ? ly * RW + lane : RW + 1;
         const uint8_t f = sf[o];
-        const T ce = cur[o];
-        if (!(f & F_FLUID)) { nxt[o] = ce; continue; }
-        const T l = (f & F_LEFT) ? cur[o - 1] : ce;
-        const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
-        const T d = (f & F_DOWN) ? cur[o - RW] : ce;
-        const T u = (f & F_UP) ? cur[o + RW] : ce;
-        nxt[o] = (T)poisson_new(l, rr, d, u, sr[o], a.alpha);
+        T ce, ns, b;
+        stencil_load(cur, sr, o, RW, f, ce, ns, b);
+        nv[k] = (f & F_FLUID) ? (T)relax_value(1, ce, ns, b, a.alpha, 1.0) : ce;
+        oo[k] = o;
+        st[k] = valid;
       }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (st[k]) nxt[oo[k]] = nv[k];
       cur = nxt;
     }
     __syncthreads();
@@ -176,6 +236,9 @@ __global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
       if (cy >= CY || cx >= CX) continue;
       const int Jc = blockIdx.y * CY + cy, Ic = blockIdx.x * CX + cx;
       if (Jc >= Hc || Ic >= Wc) continue;
+      // the four children's residuals, stored in the level's dtype
+      // (solver.py:116-117) and summed in float64 over the fluid children
+      // (transfer.py:26-33); adding +0.0 for a non-fluid child is exact
       double total = 0.0;
       int count = 0;
 #pragma unroll
@@ -184,17 +247,12 @@ __global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
         for (int di = 0; di < 2; ++di) {
           const int o = (R + 2 * cy + dj) * RW + R + 2 * cx + di;
           const uint8_t f = sf[o];
-          if (!(f & F_FLUID)) continue;
-          const T ce = cur[o];
-          const T l = (f & F_LEFT) ? cur[o - 1] : ce;
-          const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
-          const T d = (f & F_DOWN) ? cur[o - RW] : ce;
-          const T u = (f & F_UP) ? cur[o + RW] : ce;
-          // the residual is stored in the level's dtype (solver.py:116-117),
-          // then summed in float64 (transfer.py:26-33)
-          const T rv = (T)poisson_residual(l, rr, d, u, ce, sr[o], a.alpha);
-          total += (double)rv;
-          count += 1;
+          T ce, ns, b;
+          stencil_load(cur, sr, o, RW, f, ce, ns, b);
+          const T rv = (T)residual_value(ce, ns, b, a.alpha);
+          const bool fl = f & F_FLUID;
+          total += fl ? (double)rv : 0.0;
+          count += fl ? 1 : 0;
         }
       const T mean = count > 0 ? (T)(total / (double)count) : (T)0;
       a.coarse_rhs[(size_t)Jc * Wc + Ic] = (T)((double)mean / a.coarse_div);
@@ -204,20 +262,255 @@ __global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
   // ---- fused residual norm over the fluid interior cells
   if (a.norm_partial != nullptr) {
     double acc = 0.0;
+#pragma unroll 4
     for (int ly = R + wave; ly < R + TY; ly += NW) {
       const int gy = gy0 + ly;
-      if (!xout || gy >= H) continue;
-      const int o = ly * RW + lane;
+      const int o = ly * RW + (xout ? lane : R);
       const uint8_t f = sf[o];
-      if (!(f & F_FLUID)) continue;
-      const T ce = cur[o];
-      const T l = (f & F_LEFT) ? cur[o - 1] : ce;
-      const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
-      const T d = (f & F_DOWN) ? cur[o - RW] : ce;
-      const T u = (f & F_UP) ? cur[o + RW] : ce;
-      const double rv = poisson_residual(l, rr, d, u, ce, sr[o], a.alpha);
-      acc += rv * rv;
+      T ce, ns, b;
+      stencil_load(cur, sr, o, RW, f, ce, ns, b);
+      const double rv = residual_value(ce, ns, b, a.alpha);
+      acc += (xout && gy < H && (f & F_FLUID)) ? rv * rv : 0.0;
     }
+    const double tot = block_sum<NT>(acc, red);
+    if (threadIdx.x == 0) a.norm_partial[blockIdx.y * gridDim.x + blockIdx.x] = tot;
+  }
+}
+
+template <typename T, int MODE, int RH, int NT>
+__global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
+  smooth_tile64_body<T, MODE, RH, NT>(a);
+}
+
+template <typename T, int RH, int NT>
+__global__ __launch_bounds__(NT) void k_smooth_fine(const SmoothArgs<T> a) {
+  smooth_tile64_body<T, MODE_RB, RH, NT>(a);
+}
+
+// Red-black smoothing with the region in registers.  A workgroup of NW
+// wavefronts stages a 64 x (NW*BR) region; lane x owns region column x, wave
+// w owns rows [w*BR, (w+1)*BR).  Rows are held in pairs (2m, 2m+1) as
+// A[m] = the colour-0 cell and B[m] = the colour-1 cell of the pair, so a
+// colour pass updates exactly one register per pair and lane:
+//   horizontal neighbours: the other colour of lanes x-1 / x+1 (__shfl)
+//   vertical neighbours:   the other colour in this lane's own pairs
+//                          (pair m-1 / m+1), across wave bands via a 4 KB
+//                          LDS exchange of the band-edge values.
+// LDS holds only those edges, so occupancy is set by registers.  With R even
+// the 2x2 restriction blocks coincide with (lane pair, row pair).
+template <typename T>
+__device__ __forceinline__ T from_left(T v) { return __shfl_up(v, 1, 64); }
+template <typename T>
+__device__ __forceinline__ T from_right(T v) { return __shfl_down(v, 1, 64); }
+
+template <typename T>
+__device__ __forceinline__ T rb_relax(T ce, T l, T r, T d, T u, T b, uint32_t f, double alpha,
+                                      double omega) {
+  const T lv = (f & F_LEFT) ? l : ce;
+  const T rv = (f & F_RIGHT) ? r : ce;
+  const T dv = (f & F_DOWN) ? d : ce;
+  const T uv = (f & F_UP) ? u : ce;
+  const T ns = lv + rv + dv + uv;
+  return (T)relax_value(0, ce, ns, b, alpha, omega);
+}
+
+template <typename T>
+__device__ __forceinline__ T rb_resid(T ce, T l, T r, T d, T u, T b, uint32_t f, double alpha) {
+  const T lv = (f & F_LEFT) ? l : ce;
+  const T rv = (f & F_RIGHT) ? r : ce;
+  const T dv = (f & F_DOWN) ? d : ce;
+  const T uv = (f & F_UP) ? u : ce;
+  const T ns = lv + rv + dv + uv;
+  return (T)residual_value(ce, ns, b, alpha);
+}
+
+template <typename T, int NW, int BR>
+__global__ __launch_bounds__(NW * 64) void k_rb_reg(const SmoothArgs<T> a) {
+  constexpr int NT = NW * 64;
+  constexpr int RW = 64, RH = NW * BR, P = BR / 2;
+  static_assert(BR % 2 == 0, "bands hold whole row pairs");
+  __shared__ T edge[2][NW][2][64];
+  __shared__ double red[NW];
+
+  const int R = a.R;
+  const int TX = RW - 2 * R, TY = RH - 2 * R;
+  const int H = a.H, W = a.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 1;
+  const int gx0 = blockIdx.x * TX - R, gy0 = blockIdx.y * TY - R;
+  const int gx = gx0 + lane;
+  const bool colin = gx >= 0 && gx < W;
+  const int gxc = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+  const int r0 = wave * BR;
+
+  T A[P], B[P], RA[P], RB[P];
+  uint32_t FA[P], FB[P];
+  // ---- stage: unconditional clamped loads, in two halves of the band to
+  // bound the registers live at once
+#pragma unroll
+  for (int hb = 0; hb < 2; ++hb) {
+    constexpr int HR = BR / 2;
+    T pv[HR], rv[HR];
+    uint32_t fv[HR];
+#pragma unroll
+    for (int k = 0; k < HR; ++k) {
+      const int gy = gy0 + r0 + hb * HR + k;
+      const int gyc = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+      const size_t g = (size_t)gyc * W + gxc;
+      fv[k] = a.flags[g];
+      rv[k] = a.rhs[g];
+      pv[k] = a.zero_init ? (T)0 : a.p_in[g];
+    }
+    if (a.coarse_e != nullptr) {
+      T corr[HR];
+#pragma unroll
+      for (int k = 0; k < HR; ++k) {
+        const int gy = gy0 + r0 + hb * HR + k;
+        corr[k] = (T)sample_field(a.coarse_e, a.Hc, a.Wc, (gx + 0.5) * 0.5 - 0.5,
+                                  (gy + 0.5) * 0.5 - 0.5);
+      }
+#pragma unroll
+      for (int k = 0; k < HR; ++k) pv[k] = pv[k] + ((fv[k] & F_FLUID) ? corr[k] : (T)0);
+    }
+#pragma unroll
+    for (int k = 0; k < HR; ++k) {
+      const int gy = gy0 + r0 + hb * HR + k;
+      const bool in = colin && gy >= 0 && gy < H;
+      if (!in) {
+        pv[k] = (T)0;
+        rv[k] = (T)0;
+        fv[k] = 0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < HR / 2; ++q) {
+      const int m = hb * (HR / 2) + q;
+      A[m] = px ? pv[2 * q + 1] : pv[2 * q];
+      B[m] = px ? pv[2 * q] : pv[2 * q + 1];
+      RA[m] = px ? rv[2 * q + 1] : rv[2 * q];
+      RB[m] = px ? rv[2 * q] : rv[2 * q + 1];
+      FA[m] = px ? fv[2 * q + 1] : fv[2 * q];
+      FB[m] = px ? fv[2 * q] : fv[2 * q + 1];
+    }
+  }
+
+  // ---- colour passes
+  for (int pass = 0; pass < a.npass; ++pass) {
+    const int colour = (a.first_colour + pass) & 1;
+    const int lo = pass + 1;
+    const bool xin = lane >= lo && lane < RW - lo;
+    const int slot = pass & 1;
+    // publish this band's edge values of the colour read in this pass
+    edge[slot][wave][0][lane] = colour == 0 ? B[0] : A[0];
+    edge[slot][wave][1][lane] = colour == 0 ? B[P - 1] : A[P - 1];
+    __syncthreads();
+    const T below = wave > 0 ? edge[slot][wave - 1][1][lane] : (T)0;
+    const T above = wave < NW - 1 ? edge[slot][wave + 1][0][lane] : (T)0;
+    if (colour == 0) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const T bd = m > 0 ? B[m - 1] : below;
+        const T bu = m < P - 1 ? B[m + 1] : above;
+        const T l = from_left(B[m]), r = from_right(B[m]);
+        const T d = px ? B[m] : bd;
+        const T u = px ? bu : B[m];
+        const int t = r0 + 2 * m + px;
+        const bool ok = xin && t >= lo && t < RH - lo && (FA[m] & F_FLUID);
+        const T nv = rb_relax(A[m], l, r, d, u, RA[m], FA[m], a.alpha, a.omega);
+        A[m] = ok ? nv : A[m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const T ad = m > 0 ? A[m - 1] : below;
+        const T au = m < P - 1 ? A[m + 1] : above;
+        const T l = from_left(A[m]), r = from_right(A[m]);
+        const T d = px ? ad : A[m];
+        const T u = px ? A[m] : au;
+        const int t = r0 + 2 * m + 1 - px;
+        const bool ok = xin && t >= lo && t < RH - lo && (FB[m] & F_FLUID);
+        const T nv = rb_relax(B[m], l, r, d, u, RB[m], FB[m], a.alpha, a.omega);
+        B[m] = ok ? nv : B[m];
+      }
+    }
+  }
+
+  // ---- write the tile interior
+  const bool xout = lane >= R && lane < R + TX && colin;
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int ly = r0 + 2 * m + k;
+      const int gy = gy0 + ly;
+      const T v = (k == px) ? A[m] : B[m];
+      if (xout && ly >= R && ly < R + TY && gy < H) a.p_out[(size_t)gy * W + gx] = v;
+    }
+  }
+
+  if (a.coarse_rhs == nullptr && a.norm_partial == nullptr) return;
+
+  // ---- residuals of the final iterate (both colours)
+  const int slot = a.npass & 1;
+  edge[slot][wave][0][lane] = A[0];
+  edge[slot][wave][1][lane] = A[P - 1];
+  __syncthreads();
+  const T a_below = wave > 0 ? edge[slot][wave - 1][1][lane] : (T)0;
+  const T a_above = wave < NW - 1 ? edge[slot][wave + 1][0][lane] : (T)0;
+  __syncthreads();
+  edge[slot][wave][0][lane] = B[0];
+  edge[slot][wave][1][lane] = B[P - 1];
+  __syncthreads();
+  const T b_below = wave > 0 ? edge[slot][wave - 1][1][lane] : (T)0;
+  const T b_above = wave < NW - 1 ? edge[slot][wave + 1][0][lane] : (T)0;
+  const int Wc = W / 2, Hc = H / 2;
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const T bd = m > 0 ? B[m - 1] : b_below;
+    const T bu = m < P - 1 ? B[m + 1] : b_above;
+    const T ad = m > 0 ? A[m - 1] : a_below;
+    const T au = m < P - 1 ? A[m + 1] : a_above;
+    const T rA = rb_resid(A[m], from_left(B[m]), from_right(B[m]), px ? B[m] : bd,
+                          px ? bu : B[m], RA[m], FA[m], a.alpha);
+    const T rB = rb_resid(B[m], from_left(A[m]), from_right(A[m]), px ? ad : A[m],
+                          px ? A[m] : au, RB[m], FB[m], a.alpha);
+    const T rlo = px ? rB : rA;  // row 2m
+    const T rhi = px ? rA : rB;  // row 2m+1
+    const uint32_t flo = px ? FB[m] : FA[m];
+    const uint32_t fhi = px ? FA[m] : FB[m];
+    const int ly = r0 + 2 * m;
+    const int gy = gy0 + ly;
+    const bool yin = ly >= R && ly < R + TY;
+    if (a.coarse_rhs != nullptr) {
+      // 2x2 block = (lanes x, x+1) x (rows 2m, 2m+1), x even (R even); the
+      // children are summed in the reference's (dj, di) order
+      const T r01 = from_right(rlo), r11 = from_right(rhi);
+      const uint32_t f01 = (uint32_t)from_right((int)flo), f11 = (uint32_t)from_right((int)fhi);
+      double total = 0.0;
+      int count = 0;
+      total += (flo & F_FLUID) ? (double)rlo : 0.0;
+      count += (flo & F_FLUID) ? 1 : 0;
+      total += (f01 & F_FLUID) ? (double)r01 : 0.0;
+      count += (f01 & F_FLUID) ? 1 : 0;
+      total += (fhi & F_FLUID) ? (double)rhi : 0.0;
+      count += (fhi & F_FLUID) ? 1 : 0;
+      total += (f11 & F_FLUID) ? (double)r11 : 0.0;
+      count += (f11 & F_FLUID) ? 1 : 0;
+      const int Jc = gy >> 1, Ic = gx >> 1;
+      if (px == 0 && xout && yin && Jc < Hc && Ic < Wc) {
+        const T mean = count > 0 ? (T)(total / (double)count) : (T)0;
+        a.coarse_rhs[(size_t)Jc * Wc + Ic] = (T)((double)mean / a.coarse_div);
+      }
+    }
+    if (a.norm_partial != nullptr) {
+      const bool in0 = xout && yin && gy < H && (flo & F_FLUID);
+      const bool in1 = xout && yin && gy + 1 < H && (fhi & F_FLUID);
+      acc += in0 ? (double)rlo * (double)rlo : 0.0;
+      acc += in1 ? (double)rhi * (double)rhi : 0.0;
+    }
+  }
+  if (a.norm_partial != nullptr) {
     const double tot = block_sum<NT>(acc, red);
     if (threadIdx.x == 0) a.norm_partial[blockIdx.y * gridDim.x + blockIdx.x] = tot;
   }
@@ -243,42 +536,104 @@ struct SmallArgs {
   int npass_coarsest;  // coarsest level
 };
 
+// One red-black colour pass or Jacobi iteration over a whole level in LDS.
+// Levels up to 64 columns map lane -> column (row pairs for red-black, as in
+// k_smooth_tile64); new values are computed in batches of four before they
+// are stored.  Wider levels use a flat index.
 template <typename T, int MODE, int NT>
-__device__ void small_smooth(T* p, T* tmp, const T* rhs, const uint8_t* fl, int H, int W,
+__device__ __forceinline__ void small_pass(const T* cur, T* dst, const T* rhs,
+                           const uint8_t* fl, int H, int W, double alpha, double omega,
+                           int colour) {
+  constexpr int NW = NT / 64;
+  constexpr int B = 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int N = H * W;
+  // loads at a clamped index (always inside the level), selects by flags;
+  // returns whether the cell is written in this pass
+  auto update = [&](int o, T& out) -> bool {
+    const uint8_t f = fl[o];
+    const T ce = cur[o];
+    const T lv = cur[o > 0 ? o - 1 : 0];
+    const T rv = cur[o + 1 < N ? o + 1 : N - 1];
+    const T dv = cur[o >= W ? o - W : 0];
+    const T uv = cur[o + W < N ? o + W : N - 1];
+    const T b = rhs[o];
+    const T l = (f & F_LEFT) ? lv : ce;
+    const T r = (f & F_RIGHT) ? rv : ce;
+    const T d = (f & F_DOWN) ? dv : ce;
+    const T u = (f & F_UP) ? uv : ce;
+    const T ns = l + r + d + u;
+    const bool fluid = f & F_FLUID;
+    out = fluid ? (T)relax_value(MODE == MODE_RB ? 0 : 1, ce, ns, b, alpha, omega) : ce;
+    return fluid || MODE == MODE_JACOBI;  // Jacobi copies non-fluid cells
+  };
+  if (W <= 64) {
+    const bool lin = lane < W;
+    if (MODE == MODE_RB) {
+      const int par = (lane + colour) & 1;
+      const int npair = (H + 1) >> 1;
+      for (int m0 = wave; m0 < npair; m0 += B * NW) {
+        T nv[B];
+        int oo[B];
+        bool w_[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          const int j = 2 * (m0 + b * NW) + par;
+          const bool ok = lin && (m0 + b * NW) < npair && j < H;
+          oo[b] = ok ? j * W + lane : 0;
+          w_[b] = update(oo[b], nv[b]) && ok;
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+          if (w_[b]) dst[oo[b]] = nv[b];
+      }
+    } else {
+      for (int j0 = wave; j0 < H; j0 += B * NW) {
+        T nv[B];
+        int oo[B];
+        bool w_[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+          const int j = j0 + b * NW;
+          const bool ok = lin && j < H;
+          oo[b] = ok ? j * W + lane : 0;
+          w_[b] = update(oo[b], nv[b]) && ok;
+        }
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+          if (w_[b]) dst[oo[b]] = nv[b];
+      }
+    }
+    return;
+  }
+  if (MODE == MODE_RB) {
+    const int nh = (W + 1) >> 1;
+    for (int k = threadIdx.x; k < H * nh; k += NT) {
+      const int j = k / nh, c = k - j * nh;
+      const int i = ((colour + j) & 1) + 2 * c;
+      if (i >= W) continue;
+      T v;
+      if (update(j * W + i, v)) dst[j * W + i] = v;
+    }
+  } else {
+    for (int o = threadIdx.x; o < H * W; o += NT) {
+      T v;
+      if (update(o, v)) dst[o] = v;
+    }
+  }
+}
+
+template <typename T, int MODE, int NT>
+__device__ __forceinline__ void small_smooth(T* p, T* tmp, const T* rhs, const uint8_t* fl, int H, int W,
                              double alpha, double omega, int npass) {
   const int N = H * W;
   T* cur = p;
   for (int pass = 0; pass < npass; ++pass) {
     if (MODE == MODE_RB) {
-      const int colour = pass & 1;
-      const int nh = (W + 1) >> 1;
-      for (int k = threadIdx.x; k < H * nh; k += NT) {
-        const int j = k / nh, c = k - j * nh;
-        const int i = ((colour + j) & 1) + 2 * c;
-        if (i >= W) continue;
-        const int o = j * W + i;
-        const uint8_t f = fl[o];
-        if (!(f & F_FLUID)) continue;
-        const T ce = cur[o];
-        const T l = (f & F_LEFT) ? cur[o - 1] : ce;
-        const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
-        const T d = (f & F_DOWN) ? cur[o - W] : ce;
-        const T u = (f & F_UP) ? cur[o + W] : ce;
-        const double nw = poisson_new(l, rr, d, u, rhs[o], alpha);
-        cur[o] = (T)((1.0 - omega) * (double)ce + omega * nw);
-      }
+      small_pass<T, MODE, NT>(cur, cur, rhs, fl, H, W, alpha, omega, pass & 1);
     } else {
       T* nxt = (cur == p) ? tmp : p;
-      for (int o = threadIdx.x; o < N; o += NT) {
-        const uint8_t f = fl[o];
-        const T ce = cur[o];
-        if (!(f & F_FLUID)) { nxt[o] = ce; continue; }
-        const T l = (f & F_LEFT) ? cur[o - 1] : ce;
-        const T rr = (f & F_RIGHT) ? cur[o + 1] : ce;
-        const T d = (f & F_DOWN) ? cur[o - W] : ce;
-        const T u = (f & F_UP) ? cur[o + W] : ce;
-        nxt[o] = (T)poisson_new(l, rr, d, u, rhs[o], alpha);
-      }
+      small_pass<T, MODE, NT>(cur, nxt, rhs, fl, H, W, alpha, omega, 0);
       cur = nxt;
     }
     __syncthreads();
@@ -289,40 +644,155 @@ __device__ void small_smooth(T* p, T* tmp, const T* rhs, const uint8_t* fl, int 
   }
 }
 
+// Coarsest level of at most 16 x 16 cells on one wavefront with the cells in
+// registers: lane (bx, by) = (lane & 7, lane >> 3) owns cells (2bx+a, 2by+b).
+// A pass reads only values not written in that pass (the other colour, or the
+// previous Jacobi iterate), taken from the lane's own registers or from the
+// neighbouring lane with __shfl, so no pass touches LDS or a barrier.
+// Neighbour values from lanes outside the level are never used: the cell
+// flags of border cells have those neighbour bits cleared.
+template <typename T, int MODE>
+__device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t* fl, int H, int W, double alpha,
+                              double omega, int npass) {
+  const int lane = threadIdx.x & 63;
+  const int bx = lane & 7, by = lane >> 3;
+  T c[2][2], r[2][2];
+  uint8_t f[2][2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int i = 2 * bx + a, j = 2 * by + b;
+      const bool in = i < W && j < H;
+      c[b][a] = in ? p[j * W + i] : (T)0;
+      r[b][a] = in ? rhs[j * W + i] : (T)0;
+      f[b][a] = in ? fl[j * W + i] : 0;
+    }
+  const int lm = (lane - 1) & 63, lp = (lane + 1) & 63;
+  const int dm = (lane - 8) & 63, dp = (lane + 8) & 63;
+  auto relax = [&](int a, int b, T left, T right, T down, T up) -> T {
+    const uint8_t ff = f[b][a];
+    const T ce = c[b][a];
+    if (!(ff & F_FLUID)) return ce;
+    const T l = (ff & F_LEFT) ? left : ce;
+    const T rr = (ff & F_RIGHT) ? right : ce;
+    const T d = (ff & F_DOWN) ? down : ce;
+    const T u = (ff & F_UP) ? up : ce;
+    const double nw = poisson_new(l, rr, d, u, r[b][a], alpha);
+    return MODE == MODE_RB ? (T)((1.0 - omega) * (double)ce + omega * nw) : (T)nw;
+  };
+  for (int pass = 0; pass < npass; ++pass) {
+    if (MODE == MODE_RB) {
+      if ((pass & 1) == 0) {  // colour 0: cells (0,0) and (1,1)
+        const T x01 = __shfl(c[0][1], lm), x10 = __shfl(c[1][0], dm);
+        const T y10 = __shfl(c[1][0], lp), y01 = __shfl(c[0][1], dp);
+        const T n00 = relax(0, 0, x01, c[0][1], x10, c[1][0]);
+        const T n11 = relax(1, 1, c[1][0], y10, c[0][1], y01);
+        c[0][0] = n00;
+        c[1][1] = n11;
+      } else {  // colour 1: cells (1,0) and (0,1)
+        const T x00 = __shfl(c[0][0], lp), x11 = __shfl(c[1][1], dm);
+        const T y11 = __shfl(c[1][1], lm), y00 = __shfl(c[0][0], dp);
+        const T n10 = relax(1, 0, c[0][0], x00, x11, c[1][1]);
+        const T n01 = relax(0, 1, y11, c[1][1], c[0][0], y00);
+        c[0][1] = n10;
+        c[1][0] = n01;
+      }
+    } else {
+      const T l0 = __shfl(c[0][1], lm), l1 = __shfl(c[1][1], lm);
+      const T r0 = __shfl(c[0][0], lp), r1 = __shfl(c[1][0], lp);
+      const T d0 = __shfl(c[1][0], dm), d1 = __shfl(c[1][1], dm);
+      const T u0 = __shfl(c[0][0], dp), u1 = __shfl(c[0][1], dp);
+      const T n00 = relax(0, 0, l0, c[0][1], d0, c[1][0]);
+      const T n10 = relax(1, 0, c[0][0], r0, d1, c[1][1]);
+      const T n01 = relax(0, 1, l1, c[1][1], c[0][0], u0);
+      const T n11 = relax(1, 1, c[1][0], r1, c[0][1], u1);
+      c[0][0] = n00;
+      c[0][1] = n10;
+      c[1][0] = n01;
+      c[1][1] = n11;
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int i = 2 * bx + a, j = 2 * by + b;
+      if (i < W && j < H) p[j * W + i] = c[b][a];
+    }
+}
+
 template <typename T, int MODE, int NT>
 __global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  T* P[SMALL_MAX];
-  T* B[SMALL_MAX];
-  uint8_t* F[SMALL_MAX];
-  unsigned char* cursor = smem;
-  int nmax = 0;
+  // Every level array is addressed as (LDS base + per-level offset): pointers
+  // picked from an array at run time would lose the LDS address space and
+  // turn every access into a FLAT one.  Layout: [p_l, b_l]* tmp | flags_l*.
+  T* const sT = reinterpret_cast<T*>(smem);
+  int offP[SMALL_MAX], offB[SMALL_MAX], offF[SMALL_MAX];
+  int toff = 0, foff = 0, nmax = 0;
+  constexpr int PADT = 16 / (int)sizeof(T);
   for (int l = 0; l < a.nl; ++l) {
     const int n = a.H[l] * a.W[l];
+    const int np = (n + PADT - 1) / PADT * PADT;
     nmax = n > nmax ? n : nmax;
-    P[l] = reinterpret_cast<T*>(cursor);
-    cursor += ((size_t)n * sizeof(T) + 15) & ~(size_t)15;
-    B[l] = reinterpret_cast<T*>(cursor);
-    cursor += ((size_t)n * sizeof(T) + 15) & ~(size_t)15;
-    F[l] = cursor;
-    cursor += ((size_t)n + 15) & ~(size_t)15;
+    offP[l] = toff;
+    toff += np;
+    offB[l] = toff;
+    toff += np;
+    offF[l] = foff;
+    foff += (n + 15) & ~15;
   }
-  T* tmp = reinterpret_cast<T*>(cursor);  // Jacobi ping-pong partner
-  (void)nmax;
+  const int offTmp = toff;
+  if (MODE == MODE_JACOBI) toff += (nmax + PADT - 1) / PADT * PADT;
+  uint8_t* const fb = smem + (size_t)toff * sizeof(T);
+#define SP(l) (sT + offP[(l)])
+#define SB(l) (sT + offB[(l)])
+#define SF(l) (fb + offF[(l)])
+  T* const tmp = sT + offTmp;  // Jacobi ping-pong partner
 
-  // stage the top level
+  // stage the top level; batches of 8 independent loads per thread
   {
+    constexpr int BL = 8;
     const int n = a.H[0] * a.W[0];
-    for (int o = threadIdx.x; o < n; o += NT) {
-      P[0][o] = a.zero_top ? (T)0 : a.p_top_in[o];
-      B[0][o] = a.rhs_top[o];
-      F[0][o] = a.flags[0][o];
+    for (int o0 = threadIdx.x; o0 < n; o0 += BL * NT) {
+      T pv[BL], rv[BL];
+      uint8_t fv[BL];
+#pragma unroll
+      for (int b = 0; b < BL; ++b) {
+        const int o = o0 + b * NT;
+        const int oc = o < n ? o : n - 1;
+        pv[b] = a.zero_top ? (T)0 : a.p_top_in[oc];
+        rv[b] = a.rhs_top[oc];
+        fv[b] = a.flags[0][oc];
+      }
+#pragma unroll
+      for (int b = 0; b < BL; ++b) {
+        const int o = o0 + b * NT;
+        if (o < n) {
+          SP(0)[o] = pv[b];
+          SB(0)[o] = rv[b];
+          SF(0)[o] = fv[b];
+        }
+      }
     }
     for (int l = 1; l < a.nl; ++l) {
       const int nl_ = a.H[l] * a.W[l];
-      for (int o = threadIdx.x; o < nl_; o += NT) {
-        F[l][o] = a.flags[l][o];
-        P[l][o] = (T)0;  // solver.py:127
+      for (int o0 = threadIdx.x; o0 < nl_; o0 += BL * NT) {
+        uint8_t fv[BL];
+#pragma unroll
+        for (int b = 0; b < BL; ++b) {
+          const int o = o0 + b * NT;
+          fv[b] = a.flags[l][o < nl_ ? o : nl_ - 1];
+        }
+#pragma unroll
+        for (int b = 0; b < BL; ++b) {
+          const int o = o0 + b * NT;
+          if (o < nl_) {
+            SF(l)[o] = fv[b];
+            SP(l)[o] = (T)0;  // solver.py:127
+          }
+        }
       }
     }
   }
@@ -332,36 +802,46 @@ __global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
   // down
   for (int l = 0; l < last; ++l) {
     const int H = a.H[l], W = a.W[l];
-    small_smooth<T, MODE, NT>(P[l], tmp, B[l], F[l], H, W, a.alpha[l], a.omega[l],
+    small_smooth<T, MODE, NT>(SP(l), tmp, SB(l), SF(l), H, W, a.alpha[l], a.omega[l],
                               a.npass_smooth);
     const int Hc = a.H[l + 1], Wc = a.W[l + 1];
+    const int N = H * W;
     for (int k = threadIdx.x; k < Hc * Wc; k += NT) {
       const int J = k / Wc, I = k - J * Wc;
       double total = 0.0;
       int count = 0;
+#pragma unroll
       for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
         for (int di = 0; di < 2; ++di) {
           const int o = (2 * J + dj) * W + 2 * I + di;
-          const uint8_t f = F[l][o];
-          if (!(f & F_FLUID)) continue;
-          const T* p = P[l];
+          const uint8_t f = SF(l)[o];
+          const T* p = SP(l);
           const T ce = p[o];
-          const T lf = (f & F_LEFT) ? p[o - 1] : ce;
-          const T rt = (f & F_RIGHT) ? p[o + 1] : ce;
-          const T dn = (f & F_DOWN) ? p[o - W] : ce;
-          const T up = (f & F_UP) ? p[o + W] : ce;
-          const T rv = (T)poisson_residual(lf, rt, dn, up, ce, B[l][o], a.alpha[l]);
-          total += (double)rv;
-          count += 1;
+          const T lf = (f & F_LEFT) ? p[o > 0 ? o - 1 : 0] : ce;
+          const T rt = (f & F_RIGHT) ? p[o + 1 < N ? o + 1 : N - 1] : ce;
+          const T dn = (f & F_DOWN) ? p[o >= W ? o - W : 0] : ce;
+          const T up = (f & F_UP) ? p[o + W < N ? o + W : N - 1] : ce;
+          const T rv = (T)residual_value(ce, (T)(lf + rt + dn + up), SB(l)[o], a.alpha[l]);
+          const bool fl = f & F_FLUID;
+          total += fl ? (double)rv : 0.0;  // +0.0 for a non-fluid child is exact
+          count += fl ? 1 : 0;
         }
       const T mean = count > 0 ? (T)(total / (double)count) : (T)0;
-      B[l + 1][k] = (T)((double)mean / a.coarse_div[l]);
+      SB(l + 1)[k] = (T)((double)mean / a.coarse_div[l]);
     }
     __syncthreads();
   }
   // coarsest
-  small_smooth<T, MODE, NT>(P[last], tmp, B[last], F[last], a.H[last], a.W[last],
-                            a.alpha[last], a.omega[last], a.npass_coarsest);
+  if (a.H[last] <= 16 && a.W[last] <= 16) {
+    if (threadIdx.x < 64)
+      coarsest_wave<T, MODE>(SP(last), SB(last), SF(last), a.H[last], a.W[last], a.alpha[last],
+                             a.omega[last], a.npass_coarsest);
+    __syncthreads();
+  } else {
+    small_smooth<T, MODE, NT>(SP(last), tmp, SB(last), SF(last), a.H[last], a.W[last],
+                              a.alpha[last], a.omega[last], a.npass_coarsest);
+  }
   // up
   for (int l = last - 1; l >= 0; --l) {
     const int H = a.H[l], W = a.W[l];
@@ -369,16 +849,19 @@ __global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
     for (int o = threadIdx.x; o < H * W; o += NT) {
       const int j = o / W, i = o - j * W;
       T corr = (T)0;
-      if (F[l][o] & F_FLUID)
-        corr = (T)sample_field(P[l + 1], Hc, Wc, (i + 0.5) * 0.5 - 0.5, (j + 0.5) * 0.5 - 0.5);
-      P[l][o] = P[l][o] + corr;
+      if (SF(l)[o] & F_FLUID)
+        corr = (T)sample_field(SP(l + 1), Hc, Wc, (i + 0.5) * 0.5 - 0.5, (j + 0.5) * 0.5 - 0.5);
+      SP(l)[o] = SP(l)[o] + corr;
     }
     __syncthreads();
-    small_smooth<T, MODE, NT>(P[l], tmp, B[l], F[l], H, W, a.alpha[l], a.omega[l],
+    small_smooth<T, MODE, NT>(SP(l), tmp, SB(l), SF(l), H, W, a.alpha[l], a.omega[l],
                               a.npass_smooth);
   }
   const int n0 = a.H[0] * a.W[0];
-  for (int o = threadIdx.x; o < n0; o += NT) a.p_top_out[o] = P[0][o];
+  for (int o = threadIdx.x; o < n0; o += NT) a.p_top_out[o] = SP(0)[o];
+#undef SP
+#undef SB
+#undef SF
 }
 
 constexpr int NB_X = 64, NB_Y = 4, NB_T = NB_X * NB_Y;
@@ -468,35 +951,117 @@ constexpr int SMALL_NT = 256;
 inline int tile_tx(int R) { return 64 - 2 * R; }
 inline int tile_ty(int R) { return TILE_RH - 2 * R; }
 
+// ------------------------------------------------------------ profiling
+// Optional per-launch timing with HIP events recorded on the launch stream,
+// tagged with the launch's algorithmic bytes (one compulsory read or write per
+// field per logical pass: a red-black sweep reads p and rhs and writes p; a
+// residual-restriction reads p, rhs and writes a quarter-size coarse rhs; a
+// prolong-correction reads a quarter-size coarse e and reads/writes p; a norm
+// reads p and rhs).  Off by default; the benchmark turns it on for a few
+// untimed steps.
+struct ProfRec {
+  const char* name;
+  hipEvent_t start, stop;
+  double bytes;
+};
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+
+static hipEvent_t prof_begin(hipStream_t s) {
+  if (!g_prof_on) return nullptr;
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  (void)hipEventRecord(e, s);
+  return e;
+}
+
+static void prof_end(hipEvent_t start, const char* name, double bytes, hipStream_t s) {
+  if (start == nullptr) return;
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s);
+  g_prof.push_back({name, start, e, bytes});
+}
+
 template <typename T>
-int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s) {
-  if (a.R < 1 || a.R > MAX_R) return fail(SMK_EINVAL, "halo %d outside [1, %d]", a.R, MAX_R);
-  dim3 grid(cdiv(a.W, tile_tx(a.R)), cdiv(a.H, tile_ty(a.R)));
-  if (mode == MODE_RB)
-    hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, TILE_RH, TILE_NT>), grid, dim3(TILE_NT), 0,
-                       s, a);
-  else
-    hipLaunchKernelGGL((k_smooth_tile64<T, MODE_JACOBI, 32, TILE_NT>),
-                       dim3(cdiv(a.W, tile_tx(a.R)), cdiv(a.H, 32 - 2 * a.R)), dim3(TILE_NT), 0,
-                       s, a);
+double smooth_alg_bytes(int mode, const SmoothArgs<T>& a) {
+  const double es = (double)sizeof(T), n = (double)a.H * a.W;
+  double b = (mode == MODE_RB ? 0.5 * a.npass : (double)a.npass) * 3.0 * es * n;
+  if (a.coarse_rhs != nullptr) b += 2.25 * es * n;
+  if (a.coarse_e != nullptr) b += 2.25 * es * n;
+  if (a.norm_partial != nullptr) b += 2.0 * es * n;
+  return b;
+}
+
+constexpr int RB_NW = 4, RB_BR = 16, RB_RH = RB_NW * RB_BR;  // register kernel region 64 x 64
+constexpr int JAC_RH = 32;
+
+// Tile variants of the smoothing launch.  The fine level (HBM-bound) and the
+// coarse levels (latency-bound: few tiles) are tuned separately, and the fine
+// level gets its own kernel symbol so profiles separate the two.
+enum : int { VAR_LDS48 = 0, VAR_LDS32 = 1, VAR_REG64 = 2, NUM_VARIANTS = 3 };
+static int g_variant[2][2] = {{VAR_LDS48, VAR_LDS32}, {VAR_LDS48, VAR_LDS32}};  // [f32?][coarse?]
+
+inline int variant_rh(int v) { return v == VAR_LDS48 ? 48 : (v == VAR_LDS32 ? 32 : RB_RH); }
+
+template <typename T>
+int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = false) {
+  if (a.R < 1 || a.R > MAX_R + 1) return fail(SMK_EINVAL, "halo %d outside [1, %d]", a.R, MAX_R + 1);
+  hipEvent_t ev = prof_begin(s);
+  const bool f32 = sizeof(T) == 4;
+  const char* tag = mode == MODE_RB
+                        ? (fine ? (f32 ? "rb_smooth_fine_f32" : "rb_smooth_fine_f64")
+                                : (f32 ? "rb_smooth_coarse_f32" : "rb_smooth_coarse_f64"))
+                        : (fine ? (f32 ? "jacobi_fine_f32" : "jacobi_fine_f64")
+                                : (f32 ? "jacobi_coarse_f32" : "jacobi_coarse_f64"));
+  struct Done {
+    hipEvent_t ev;
+    const char* tag;
+    double bytes;
+    hipStream_t s;
+    ~Done() { prof_end(ev, tag, bytes, s); }
+  } done{ev, tag, ev ? smooth_alg_bytes<T>(mode, a) : 0.0, s};
+  if (mode == MODE_RB) {
+    const int v = g_variant[sizeof(T) == 4][fine ? 0 : 1];
+    const int rh = variant_rh(v);
+    if (v == VAR_REG64 && a.coarse_rhs != nullptr && (a.R & 1))
+      return fail(SMK_EINVAL, "restriction needs an even halo (got %d)", a.R);
+    dim3 grid(cdiv(a.W, tile_tx(a.R)), cdiv(a.H, rh - 2 * a.R));
+    if (v == VAR_REG64)
+      hipLaunchKernelGGL((k_rb_reg<T, RB_NW, RB_BR>), grid, dim3(RB_NW * 64), 0, s, a);
+    else if (v == VAR_LDS32)
+      hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, 32, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
+    else if (fine)
+      hipLaunchKernelGGL((k_smooth_fine<T, 48, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, 48, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
+    return check_launch("red-black tile kernel");
+  }
+  hipLaunchKernelGGL((k_smooth_tile64<T, MODE_JACOBI, JAC_RH, TILE_NT>),
+                     dim3(cdiv(a.W, tile_tx(a.R)), cdiv(a.H, JAC_RH - 2 * a.R)), dim3(TILE_NT), 0,
+                     s, a);
   return check_launch("k_smooth_tile64");
 }
 
 // partial-sum slots one tiled launch can produce
-inline int tiled_blocks(int mode, int H, int W, int R) {
-  const int ty = mode == MODE_RB ? tile_ty(R) : 32 - 2 * R;
-  return (int)(cdiv(W, tile_tx(R)) * cdiv(H, ty));
+inline int tiled_blocks(int mode, int H, int W, int R, bool fine = false, bool f32 = false) {
+  const int rh = mode == MODE_RB ? variant_rh(g_variant[f32][fine ? 0 : 1]) : JAC_RH;
+  return (int)(cdiv(W, tile_tx(R)) * cdiv(H, rh - 2 * R));
 }
 
 inline size_t small_lds_bytes(const int* Hs, const int* Ws, int nl, size_t es, int mode) {
-  size_t b = 0, nmax = 0;
+  // must match the layout computed in k_vcycle_small
+  const size_t padt = 16 / es;
+  size_t t = 0, f = 0, nmax = 0;
   for (int l = 0; l < nl; ++l) {
     const size_t n = (size_t)Hs[l] * Ws[l];
+    const size_t np = (n + padt - 1) / padt * padt;
     nmax = n > nmax ? n : nmax;
-    b += 2 * ((n * es + 15) & ~(size_t)15) + ((n + 15) & ~(size_t)15);
+    t += 2 * np;
+    f += (n + 15) & ~(size_t)15;
   }
-  if (mode == MODE_JACOBI) b += nmax * es;
-  return b;
+  if (mode == MODE_JACOBI) t += (nmax + padt - 1) / padt * padt;
+  return t * es + f;
 }
 
 static size_t g_lds_limit = 0;
@@ -533,6 +1098,24 @@ static size_t small_lds_limit() {
 
 template <typename T>
 int launch_small(int mode, const SmallArgs<T>& a, size_t lds, hipStream_t s) {
+  hipEvent_t ev = prof_begin(s);
+  double bytes = 0.0;
+  if (ev) {
+    const double es = (double)sizeof(T);
+    const double sweeps = mode == MODE_RB ? 0.5 : 1.0;
+    for (int l = 0; l < a.nl; ++l) {
+      const double n = (double)a.H[l] * a.W[l];
+      const bool last = l == a.nl - 1;
+      bytes += (last ? a.npass_coarsest : 2 * a.npass_smooth) * sweeps * 3.0 * es * n;
+      if (!last) bytes += 4.5 * es * n;
+    }
+  }
+  struct Done {
+    hipEvent_t ev;
+    double bytes;
+    hipStream_t s;
+    ~Done() { prof_end(ev, sizeof(T) == 4 ? "vcycle_small_f32" : "vcycle_small_f64", bytes, s); }
+  } done{ev, bytes, s};
   if (mode == MODE_RB)
     hipLaunchKernelGGL((k_vcycle_small<T, MODE_RB, SMALL_NT>), dim3(1), dim3(SMALL_NT), lds, s,
                        a);
@@ -644,7 +1227,7 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
     a.R = chunks[q];
     a.zero_init = 0;
     a.coarse_div = 1.0;
-    int rc = launch_tiled<T>(mode, a, s);
+    int rc = launch_tiled<T>(mode, a, s, true);
     if (rc) return rc;
     colour = (colour + chunks[q]) & 1;
     in = out;
@@ -657,8 +1240,11 @@ int residual_norm2_dev(const T* p, const T* rhs, const uint8_t* flags, int H, in
                        double alpha, double* partial, double* result, int slot, float* defect,
                        hipStream_t s) {
   dim3 g(cdiv(W, NB_X), cdiv(H, NB_Y));
+  hipEvent_t ev = prof_begin(s);
   hipLaunchKernelGGL(k_residual_norm<T>, g, dim3(NB_T), 0, s, p, rhs, flags, H, W, alpha,
                      partial, defect);
+  prof_end(ev, defect ? "defect_fp64" : "residual_norm",
+           (2.0 * sizeof(T) + (defect ? 4.0 : 0.0)) * (double)H * W, s);
   int rc = check_launch("k_residual_norm");
   if (rc) return rc;
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partial, (int)(g.x * g.y),
@@ -840,6 +1426,10 @@ int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_in
     a.first_colour = colour;
     a.R = chunks[q] + ((last && tail) ? 1 : 0);
     if (a.R < 1) a.R = 1;
+    // the register red-black variant restricts (lane pair, row pair) blocks
+    if (mode == MODE_RB && last && coarse_rhs != nullptr &&
+        g_variant[sizeof(T) == 4][l == 0 ? 0 : 1] == VAR_REG64)
+      a.R += a.R & 1;
     a.zero_init = (first && zero_init) ? 1 : 0;
     if (first && coarse_e != nullptr) {
       a.coarse_e = coarse_e;
@@ -853,9 +1443,9 @@ int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_in
     }
     if (last && norm_partial != nullptr) {
       a.norm_partial = norm_partial;
-      *nparts = tiled_blocks(mode, L.H, L.W, a.R);
+      *nparts = tiled_blocks(mode, L.H, L.W, a.R, l == 0, sizeof(T) == 4);
     }
-    int rc = launch_tiled<T>(mode, a, s);
+    int rc = launch_tiled<T>(mode, a, s, l == 0);
     if (rc) return rc;
     cur ^= 1;
     colour = (colour + chunks[q]) & 1;
@@ -1074,7 +1664,9 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
       inner_total++;
     }
     const float* e = reinterpret_cast<const float*>(L0.p32[L0.cur32]);
+    hipEvent_t ev = prof_begin(s);
     hipLaunchKernelGGL(k_add_correction<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, e, n);
+    prof_end(ev, "correct_fp64", 20.0 * (double)n, s);
     rc = check_launch("k_add_correction");
     if (rc) return rc;
     refinements++;
@@ -1094,6 +1686,56 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
 using namespace smk;
 
 extern "C" {
+
+int smk_profile_enable(int on) {
+  g_prof_on = on != 0;
+  return SMK_OK;
+}
+
+int smk_profile_read(smk_profile_record* out, int n, int reset) {
+  // aggregate by tag in first-seen order
+  std::vector<smk_profile_record> agg;
+  for (ProfRec& r : g_prof) {
+    hipError_t e = hipEventSynchronize(r.stop);
+    if (e != hipSuccess) return fail((int)e, "smk_profile_read: %s", hipGetErrorString(e));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, r.start, r.stop);
+    size_t k = 0;
+    while (k < agg.size() && strncmp(agg[k].name, r.name, sizeof(agg[k].name)) != 0) ++k;
+    if (k == agg.size()) {
+      smk_profile_record z{};
+      strncpy(z.name, r.name, sizeof(z.name) - 1);
+      agg.push_back(z);
+    }
+    agg[k].launches += 1;
+    agg[k].total_ms += ms;
+    agg[k].alg_bytes += r.bytes;
+  }
+  if (reset) {
+    for (ProfRec& r : g_prof) {
+      (void)hipEventDestroy(r.start);
+      (void)hipEventDestroy(r.stop);
+    }
+    g_prof.clear();
+  }
+  const int m = (int)agg.size() < n ? (int)agg.size() : n;
+  for (int k = 0; k < m && out != nullptr; ++k) out[k] = agg[k];
+  return (int)agg.size();
+}
+
+int smk_set_smoother_variant(int dtype, int which, int variant) {
+  SMK_REQUIRE(dtype == SMK_F64 || dtype == SMK_F32, "smk_set_smoother_variant: bad dtype");
+  SMK_REQUIRE(which == 0 || which == 1, "smk_set_smoother_variant: which is 0 (fine) or 1 (coarse)");
+  SMK_REQUIRE(variant >= 0 && variant < NUM_VARIANTS, "smk_set_smoother_variant: bad variant %d",
+              variant);
+  g_variant[dtype == SMK_F32][which] = variant;
+  return SMK_OK;
+}
+
+int smk_get_smoother_variant(int dtype, int which) {
+  if ((dtype != SMK_F64 && dtype != SMK_F32) || (which != 0 && which != 1)) return -1;
+  return g_variant[dtype == SMK_F32][which];
+}
 
 size_t smk_solver_workspace_bytes(int H, int W, int dtype) {
   if (H <= 0 || W <= 0) return 0;
