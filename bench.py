@@ -77,10 +77,11 @@ def profile_steps(sim, scene, n: int) -> list:
     return sorted(recs, key=lambda r: -r["total_ms"])
 
 
-def calibrate_copy(nbytes: int = 256 << 20) -> float:
+def calibrate_copy(nbytes: int = 1 << 30) -> float:
     """A plain device copy of known size (nbytes read + nbytes written), run
     once so that HBM byte counters (FETCH_SIZE / WRITE_SIZE) of a profiled run
-    can be scaled; returns its GB/s."""
+    can be scaled; returns its GB/s.  1 GiB per buffer is far above the
+    256 MiB last-level (MALL) cache, so the counters see HBM traffic."""
     import torch
 
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)

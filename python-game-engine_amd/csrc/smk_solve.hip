@@ -128,38 +128,43 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a) {
   // per row behind a bounds branch.  Out-of-grid cells are then zeroed.
   static_assert(RH % (2 * NW) == 0, "row pairs must split evenly over the waves");
   constexpr int KR = RH / NW;
+  // staged in two halves to bound the registers held by loads in flight
+  constexpr int KH = (KR % 2 == 0) ? KR / 2 : KR;
   const int gxc = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
-  T pv[KR], rv[KR];
-  uint8_t fv[KR];
 #pragma unroll
-  for (int k = 0; k < KR; ++k) {
-    const int gy = gy0 + wave + k * NW;
-    const int gyc = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
-    const size_t g = (size_t)gyc * W + gxc;
-    fv[k] = a.flags[g];
-    rv[k] = a.rhs[g];
-    pv[k] = a.zero_init ? (T)0 : a.p_in[g];
-  }
-  if (a.coarse_e != nullptr) {
-    T corr[KR];
+  for (int h0 = 0; h0 < KR; h0 += KH) {
+    T pv[KH], rv[KH];
+    uint8_t fv[KH];
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      const int gy = gy0 + wave + k * NW;
-      const double xc = (gx + 0.5) * 0.5, yc = (gy + 0.5) * 0.5;
-      corr[k] = (T)sample_field(a.coarse_e, a.Hc, a.Wc, xc - 0.5, yc - 0.5);
+    for (int k = 0; k < KH; ++k) {
+      const int gy = gy0 + wave + (h0 + k) * NW;
+      const int gyc = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+      const size_t g = (size_t)gyc * W + gxc;
+      fv[k] = a.flags[g];
+      rv[k] = a.rhs[g];
+      pv[k] = a.zero_init ? (T)0 : a.p_in[g];
+    }
+    if (a.coarse_e != nullptr) {
+      T corr[KH];
+#pragma unroll
+      for (int k = 0; k < KH; ++k) {
+        const int gy = gy0 + wave + (h0 + k) * NW;
+        const double xc = (gx + 0.5) * 0.5, yc = (gy + 0.5) * 0.5;
+        corr[k] = (T)sample_field(a.coarse_e, a.Hc, a.Wc, xc - 0.5, yc - 0.5);
+      }
+#pragma unroll
+      for (int k = 0; k < KH; ++k) pv[k] = pv[k] + ((fv[k] & F_FLUID) ? corr[k] : (T)0);
     }
 #pragma unroll
-    for (int k = 0; k < KR; ++k) pv[k] = pv[k] + ((fv[k] & F_FLUID) ? corr[k] : (T)0);
-  }
-#pragma unroll
-  for (int k = 0; k < KR; ++k) {
-    const int ly = wave + k * NW;
-    const int gy = gy0 + ly;
-    const bool in = colin && gy >= 0 && gy < H;
-    const int o = ly * RW + lane;
-    sp[o] = in ? pv[k] : (T)0;
-    sr[o] = in ? rv[k] : (T)0;
-    sf[o] = in ? fv[k] : (uint8_t)0;
+    for (int k = 0; k < KH; ++k) {
+      const int ly = wave + (h0 + k) * NW;
+      const int gy = gy0 + ly;
+      const bool in = colin && gy >= 0 && gy < H;
+      const int o = ly * RW + lane;
+      sp[o] = in ? pv[k] : (T)0;
+      sr[o] = in ? rv[k] : (T)0;
+      sf[o] = in ? fv[k] : (uint8_t)0;
+    }
   }
   __syncthreads();
 
