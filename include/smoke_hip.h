@@ -221,6 +221,71 @@ int smk_mg_solve(smk_mg* mg, void* p, const void* div, double fluid_density,
                  smk_solve_info* info, void* stream);
 int smk_mg_destroy(smk_mg* mg);
 
+/* ------------------------------------------- differentiable transport */
+/* No reference counterpart (the reference has no gradients): forward pieces
+ * whose controls live in device memory, and the adjoints (transposed
+ * Jacobians) of the transport operators of SmokeSimulation2D.substep
+ * (src/cfd/common/simulation.py:206-249).  u is H x (W+1), v (H+1) x W.
+ * The projection's adjoint needs no entry point: it is the forward
+ * divergence / solve / gradient applied after smk_enforce_boundaries.
+ *
+ * Adjoint modes for the transposed bilinear gather of advection:
+ *   SMK_ADJ_GATHER   deterministic: each source visits the targets that can
+ *                    reach it (window = max backtrace displacement + 1)
+ *   SMK_ADJ_SCATTER  each target adds its weights with atomics
+ * Where a backtrace lies exactly on a grid line the slope used is the mean
+ * of the one-sided slopes (what a central difference measures). */
+#define SMK_ADJ_GATHER 0
+#define SMK_ADJ_SCATTER 1
+
+/* Bytes of scratch the two advection adjoints need for an H x W grid. */
+size_t smk_adjoint_workspace_bytes(int H, int W, int dtype);
+
+/* field[y0:y1, x0:x1] = values[index] (a device array): the box source with
+ * a strength that is a control. */
+int smk_box_set_indexed(void* field, int H, int W, int x0, int y0, int x1,
+                        int y1, const void* values, int index, int dtype,
+                        void* stream);
+/* Adjoint of smk_box_set_indexed: values_bar[index] += sum of
+ * field_bar over the box, then field_bar[box] = 0. */
+int smk_box_set_adjoint(void* field_bar, int H, int W, int x0, int y0, int x1,
+                        int y1, void* values_bar, int index, int dtype,
+                        void* stream);
+
+/* forces.py:19-46 buoyancy (temperature term 0) with the force kept in the
+ * field type instead of the reference's float32 scratch (forces.py:33), so
+ * that it is differentiable: v[j] += dt * (-dc * rho_face(j) * g), j < H. */
+int smk_apply_buoyancy(void* v, const void* rho, int H, int W, double dt,
+                       double gravity, double density_coefficient, int dtype,
+                       void* stream);
+/* rho_bar += (d v / d rho)^T v_bar for smk_apply_buoyancy. */
+int smk_apply_buoyancy_adjoint(const void* v_bar, void* rho_bar, int H, int W,
+                               double dt, double gravity,
+                               double density_coefficient, int dtype,
+                               void* stream);
+
+/* Adjoint of smk_advect_scalar (advection.py:112-128, scale 1) for the
+ * gradient dst_bar of its output: src_bar = (d dst / d src)^T dst_bar
+ * (overwritten); when u_bar / v_bar are given, the backtrace sensitivity
+ * (d dst / d u, d v)^T dst_bar is added to them. */
+int smk_advect_scalar_adjoint(const void* dst_bar, const void* src,
+                              const void* u, const void* v,
+                              const uint8_t* is_fluid, int H, int W,
+                              double dt, double cell_size, void* src_bar,
+                              void* u_bar, void* v_bar, int mode,
+                              void* workspace, size_t workspace_bytes,
+                              int dtype, void* stream);
+/* Adjoint of smk_advect_velocity (advection.py:130-180: u and v both
+ * advected by the old velocity (us, vs)): us_bar, vs_bar (overwritten) =
+ * (d (ud, vd) / d (us, vs))^T (ud_bar, vd_bar). */
+int smk_advect_velocity_adjoint(const void* ud_bar, const void* vd_bar,
+                                const void* us, const void* vs,
+                                const uint8_t* fmu, const uint8_t* fmv, int H,
+                                int W, double dt, double cell_size,
+                                void* us_bar, void* vs_bar, int mode,
+                                void* workspace, size_t workspace_bytes,
+                                int dtype, void* stream);
+
 /* ------------------------------------------------- profiling / tuning */
 /* No reference counterpart: measurement and tuning hooks.
  * smk_profile_enable(1) makes every solver launch record a HIP event pair on

@@ -23,6 +23,8 @@ SMK_COARSE_REFERENCE = 0
 SMK_COARSE_REDISCRETIZED = 1
 SMK_PREC_NATIVE = 0
 SMK_PREC_MIXED = 1
+SMK_ADJ_GATHER = 0
+SMK_ADJ_SCATTER = 1
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -103,6 +105,15 @@ SIGNATURES = {
     "smk_profile_read": (_i, [ctypes.POINTER(ProfileRecord), _i, _i]),
     "smk_set_smoother_variant": (_i, [_i, _i, _i]),
     "smk_get_smoother_variant": (_i, [_i, _i]),
+    "smk_adjoint_workspace_bytes": (_sz, [_i, _i, _i]),
+    "smk_box_set_indexed": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _i, _i, _vp]),
+    "smk_box_set_adjoint": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _i, _i, _vp]),
+    "smk_apply_buoyancy": (_i, [_vp, _vp, _i, _i, _d, _d, _d, _i, _vp]),
+    "smk_apply_buoyancy_adjoint": (_i, [_vp, _vp, _i, _i, _d, _d, _d, _i, _vp]),
+    "smk_advect_scalar_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp, _vp, _i,
+                                       _vp, _sz, _i, _vp]),
+    "smk_advect_velocity_adjoint": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _vp, _i,
+                                         _vp, _sz, _i, _vp]),
 }
 
 _lib = None
