@@ -286,6 +286,74 @@ int smk_advect_velocity_adjoint(const void* ud_bar, const void* vd_bar,
                                 void* workspace, size_t workspace_bytes,
                                 int dtype, void* stream);
 
+/* ------------------------------------------------------ batched scenes */
+/* No reference counterpart (the reference steps one simulation at a time):
+ * many independent plume scenes of uneven sizes advanced together.  Each
+ * step is SmokeSimulation2D.substep (simulation.py:186-249) per scene with
+ * a box source (density := box_value), buoyancy (forces.py:78-105, no
+ * temperature), RK1 advection, the divergence, a multigrid solve of FP32
+ * V-cycles inside FP64 defect correction to `tolerance` (per scene), and
+ * the pressure gradient with boundaries; the per-scene results are the
+ * ones the single-scene entry points give.  All scenes' tiles go through
+ * one persistent launch per stage; with use_graphs the stages are replayed
+ * from HIP graphs.  Fields are float64 and owned by the caller. */
+typedef struct smk_batch_scene {
+  int H, W;
+  double cell_size;
+  int num_levels;              /* multigrid levels (halving while even)   */
+  void* density[2];            /* ping-pong; density[0] is current first */
+  void* u;                     /* H x (W+1) state                         */
+  void* u_tmp;                 /* H x (W+1) scratch (advected u)          */
+  void* v;                     /* (H+1) x W state                         */
+  void* v_tmp;                 /* (H+1) x W scratch                       */
+  void* pressure;              /* H x W                                   */
+  void* divergence;            /* H x W                                   */
+  const uint8_t* face_mask_u;
+  const uint8_t* face_mask_v;
+  const uint8_t* is_fluid;
+  const uint8_t* solid;        /* boundary enforcement mask               */
+  int box_x0, box_y0, box_x1, box_y1;
+  double box_value;
+} smk_batch_scene;
+
+typedef struct smk_batch_params {
+  double dt, gravity, buoyancy, density_coefficient, ambient_temperature;
+  double fluid_density;
+  int smoother_iterations;     /* V(s,s), 1..3                            */
+  int coarsest_iterations;
+  int coarse_scale;            /* SMK_COARSE_*                            */
+  double tolerance;            /* relative residual per scene             */
+  int max_cycles;              /* V-cycles per scene per step             */
+  int use_graphs;              /* replay stages from HIP graphs           */
+  int tile_rows;               /* smoother tile height: 48 or 32          */
+  int persistent_blocks;       /* workgroups per persistent launch (0: auto) */
+} smk_batch_params;
+
+typedef struct smk_batch_info {
+  int iterations;              /* refinement iterations of the slowest scene */
+  int cycles_total;            /* V-cycles summed over scenes             */
+  double max_rel_residual;     /* worst scene's final relative residual   */
+} smk_batch_info;
+
+typedef struct smk_batch smk_batch;
+int smk_batch_create(const smk_batch_scene* scenes, int n,
+                     const smk_batch_params* params, void* stream,
+                     smk_batch** out);
+/* One step of every scene.  Reads a 4-byte active-scene count back once
+ * per refinement iteration; fills info when non-NULL. */
+int smk_batch_step(smk_batch* b, smk_batch_info* info, void* stream);
+/* Which density buffer (0 or 1) holds the current density. */
+int smk_batch_density_index(const smk_batch* b);
+/* Per-scene V-cycles and relative residual of the last step (n entries,
+ * either may be NULL; synchronises the stream). */
+int smk_batch_scene_stats(const smk_batch* b, int* cycles, double* rel,
+                          void* stream);
+/* Change tile height (48/32), V(s,s) sweeps (1..3) and persistent grid
+ * size; rebuilds the job lists and graphs. */
+int smk_batch_set_tuning(smk_batch* b, int tile_rows, int smoother_iterations,
+                         int persistent_blocks, void* stream);
+int smk_batch_destroy(smk_batch* b);
+
 /* ------------------------------------------------- profiling / tuning */
 /* No reference counterpart: measurement and tuning hooks.
  * smk_profile_enable(1) makes every solver launch record a HIP event pair on

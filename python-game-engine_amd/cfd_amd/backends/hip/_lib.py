@@ -69,6 +69,36 @@ class MgParams(ctypes.Structure):
     ]
 
 
+class BatchScene(ctypes.Structure):
+    _fields_ = [
+        ("H", ctypes.c_int), ("W", ctypes.c_int), ("cell_size", ctypes.c_double),
+        ("num_levels", ctypes.c_int),
+        ("density", ctypes.c_void_p * 2), ("u", ctypes.c_void_p), ("u_tmp", ctypes.c_void_p),
+        ("v", ctypes.c_void_p), ("v_tmp", ctypes.c_void_p), ("pressure", ctypes.c_void_p),
+        ("divergence", ctypes.c_void_p), ("face_mask_u", ctypes.c_void_p),
+        ("face_mask_v", ctypes.c_void_p), ("is_fluid", ctypes.c_void_p), ("solid", ctypes.c_void_p),
+        ("box_x0", ctypes.c_int), ("box_y0", ctypes.c_int), ("box_x1", ctypes.c_int),
+        ("box_y1", ctypes.c_int), ("box_value", ctypes.c_double),
+    ]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double), ("gravity", ctypes.c_double), ("buoyancy", ctypes.c_double),
+        ("density_coefficient", ctypes.c_double), ("ambient_temperature", ctypes.c_double),
+        ("fluid_density", ctypes.c_double), ("smoother_iterations", ctypes.c_int),
+        ("coarsest_iterations", ctypes.c_int), ("coarse_scale", ctypes.c_int),
+        ("tolerance", ctypes.c_double), ("max_cycles", ctypes.c_int),
+        ("use_graphs", ctypes.c_int), ("tile_rows", ctypes.c_int),
+        ("persistent_blocks", ctypes.c_int),
+    ]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("cycles_total", ctypes.c_int),
+                ("max_rel_residual", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/smoke_hip.h
 SIGNATURES = {
     "smk_last_error": (ctypes.c_char_p, []),
@@ -106,6 +136,14 @@ SIGNATURES = {
     "smk_set_smoother_variant": (_i, [_i, _i, _i]),
     "smk_get_smoother_variant": (_i, [_i, _i]),
     "smk_adjoint_workspace_bytes": (_sz, [_i, _i, _i]),
+    "smk_batch_create": (_i, [ctypes.POINTER(BatchScene), _i, ctypes.POINTER(BatchParams), _vp,
+                              ctypes.POINTER(_vp)]),
+    "smk_batch_step": (_i, [_vp, ctypes.POINTER(BatchInfo), _vp]),
+    "smk_batch_density_index": (_i, [_vp]),
+    "smk_batch_scene_stats": (_i, [_vp, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_double), _vp]),
+    "smk_batch_set_tuning": (_i, [_vp, _i, _i, _i, _vp]),
+    "smk_batch_destroy": (_i, [_vp]),
     "smk_box_set_indexed": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _i, _i, _vp]),
     "smk_box_set_adjoint": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _i, _i, _vp]),
     "smk_apply_buoyancy": (_i, [_vp, _vp, _i, _i, _d, _d, _d, _i, _vp]),

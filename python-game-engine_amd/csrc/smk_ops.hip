@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <mutex>
 
+#include "smk_batch.hpp"
 #include "smk_common.hpp"
 
 namespace smk {
@@ -364,6 +365,223 @@ __global__ void k_box(T* __restrict__ f, int W, int x0, int y0, int x1, int y1, 
   if (x >= x1 || y >= y1) return;
   const size_t o = (size_t)y * W + x;
   f[o] = mode == 0 ? value : f[o] + value;
+}
+
+// ------------------------------------------------------ batched transport
+// The same per-face / per-cell arithmetic as the kernels above, for many
+// scenes in one persistent launch: workgroups walk a packed job list of
+// 64 x 4 tiles ({scene, tile x, tile y, -}); a thread owns one position of
+// the tile.  Face jobs tile the (H+1) x (W+1) face superset of their scene.
+
+__device__ __forceinline__ void job_pos(const int4& jb, int& i, int& j) {
+  i = jb.y * BT_X + (int)(threadIdx.x & (BT_X - 1));
+  j = jb.z * BT_Y + (int)(threadIdx.x / BT_X);
+}
+
+__device__ __forceinline__ bool in_box(const BatchScene& d, int j, int i) {
+  return i >= d.box_x0 && i < d.box_x1 && j >= d.box_y0 && j < d.box_y1;
+}
+
+// Box source (density := value in the box) fused with forces.py:78-105: the
+// force reads the density with the box applied, so no tile depends on
+// another tile's writes.
+__global__ __launch_bounds__(BT_NT) void k_b_box_forces(const BatchScene* __restrict__ sc,
+                                                        const int4* __restrict__ jobs, int njobs,
+                                                        const int* __restrict__ parity,
+                                                        BatchPhys ph) {
+  const int par = *parity;
+  for (int q = blockIdx.x; q < njobs; q += gridDim.x) {
+    const int4 jb = jobs[q];
+    const BatchScene& d = sc[jb.x];
+    const int H = d.H, W = d.W;
+    int i, j;
+    job_pos(jb, i, j);
+    double* rho = d.rho[par];
+    if (j < H && i < W && in_box(d, j, i)) rho[(size_t)j * W + i] = d.box_value;
+    if (j > H || i >= W) continue;
+    auto rho_at = [&](int r) -> double {
+      return in_box(d, r, i) ? d.box_value : rho[(size_t)r * W + i];
+    };
+    float bf = 0.0f;
+    if (j < H) {
+      double dd;
+      if (j == 0 || j == H - 1) {
+        dd = rho_at(j);
+      } else {
+        const double s2 = rho_at(j - 1) + rho_at(j);
+        dd = 0.5 * s2;
+      }
+      bf = (float)(-ph.density_coefficient * dd * ph.gravity);
+    }
+    const float tf = (float)(-ph.buoyancy * (0.0 - ph.ambient_temperature));
+    const float s = bf + tf;
+    const size_t o = (size_t)j * W + i;
+    d.v[o] = d.v[o] + ph.dt * (double)s;
+  }
+}
+
+__global__ __launch_bounds__(BT_NT) void k_b_advect_velocity(const BatchScene* __restrict__ sc,
+                                                             const int4* __restrict__ jobs,
+                                                             int njobs, BatchPhys ph) {
+  for (int q = blockIdx.x; q < njobs; q += gridDim.x) {
+    const int4 jb = jobs[q];
+    const BatchScene& d = sc[jb.x];
+    const int H = d.H, W = d.W;
+    const double inv = 1.0 / d.h;
+    int i, j;
+    job_pos(jb, i, j);
+    if (j < H && i <= W) advect_u_at(d.u, d.u_tmp, H, W + 1, d.v, H + 1, W, d.fmu, ph.dt, inv, j, i);
+    if (j <= H && i < W) advect_v_at(d.v, d.v_tmp, H + 1, W, d.u, H, W + 1, d.fmv, ph.dt, inv, j, i);
+  }
+}
+
+__global__ __launch_bounds__(BT_NT) void k_b_advect_density(const BatchScene* __restrict__ sc,
+                                                            const int4* __restrict__ jobs,
+                                                            int njobs,
+                                                            const int* __restrict__ parity,
+                                                            BatchPhys ph) {
+  const int par = *parity;
+  for (int q = blockIdx.x; q < njobs; q += gridDim.x) {
+    const int4 jb = jobs[q];
+    const BatchScene& d = sc[jb.x];
+    const int H = d.H, W = d.W;
+    int i, j;
+    job_pos(jb, i, j);
+    if (j >= H || i >= W) continue;
+    const size_t o = (size_t)j * W + i;
+    const double* src = d.rho[par];
+    double* dst = d.rho[par ^ 1];
+    if (!d.fluid[o]) {
+      dst[o] = 0.0;
+      continue;
+    }
+    const double inv = 1.0 / d.h;
+    const double px = i + 0.5, py = j + 0.5;
+    const double uu = sample_field(d.u_tmp, H, W + 1, px, py - 0.5);
+    const double vv = sample_field(d.v_tmp, H + 1, W, px - 0.5, py);
+    const double x = px - (uu * ph.dt * inv);
+    const double y = py - (vv * ph.dt * inv);
+    dst[o] = sample_field(src, H, W, x - 0.5, y - 0.5);
+  }
+}
+
+// divergence.py:4-25 of the advected velocity, p := 0 (simulation.py:310)
+// and the ||alpha*div||^2 partial of the tile (the tile is the 64 x 4 block
+// of the single-scene norm kernel, so the partials are the same numbers).
+__global__ __launch_bounds__(BT_NT) void k_b_divergence_norm(const BatchScene* __restrict__ sc,
+                                                             const int4* __restrict__ jobs,
+                                                             int njobs,
+                                                             double* __restrict__ partial) {
+  __shared__ double red[BT_NT / 64];
+  for (int q = blockIdx.x; q < njobs; q += gridDim.x) {
+    const int4 jb = jobs[q];
+    const BatchScene& d = sc[jb.x];
+    const int H = d.H, W = d.W;
+    const double inv = 1.0 / d.h;
+    int i, j;
+    job_pos(jb, i, j);
+    double acc = 0.0;
+    if (j < H && i < W) {
+      const size_t a = (size_t)j * (W + 1) + i;
+      const size_t b = (size_t)j * W + i;
+      const double ddu = (d.u_tmp[a + 1] * (double)d.fmu[a + 1]) - (d.u_tmp[a] * (double)d.fmu[a]);
+      const double ddv = (d.v_tmp[b + W] * (double)d.fmv[b + W]) - (d.v_tmp[b] * (double)d.fmv[b]);
+      const double du = ddu * inv;
+      const double dv = ddv * inv;
+      const double dvg = du + dv;
+      d.div[b] = dvg;
+      d.p[b] = 0.0;
+      if (d.flags0[b] & F_FLUID) {
+        const double bb = d.alpha0 * dvg;
+        acc = bb * bb;
+      }
+    }
+    const double tot = block_sum<BT_NT>(acc, red);
+    if (threadIdx.x == 0) partial[d.part_base + jb.z * d.nbx + jb.y] = tot;
+  }
+}
+
+// pressure_gradient.py:4-37 + boundaries.py:7-39 from the advected velocity
+// into the state; flips the density parity for the next step.
+__global__ __launch_bounds__(BT_NT) void k_b_gradient(const BatchScene* __restrict__ sc,
+                                                      const int4* __restrict__ jobs, int njobs,
+                                                      int* __restrict__ parity, BatchPhys ph) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *parity ^= 1;
+  for (int q = blockIdx.x; q < njobs; q += gridDim.x) {
+    const int4 jb = jobs[q];
+    const BatchScene& d = sc[jb.x];
+    const int H = d.H, W = d.W;
+    const double scale = ph.dt / (ph.fluid_density * d.h);
+    int i, j;
+    job_pos(jb, i, j);
+    const double* p = d.p;
+    if (j < H && i <= W) {
+      const size_t o = (size_t)j * (W + 1) + i;
+      double val = 0.0;
+      if (d.fmu[o]) {
+        const int ir = i < W ? i : W - 1;
+        const int il = i - 1 < 0 ? W - 1 : i - 1;
+        const double dp = p[(size_t)j * W + ir] - p[(size_t)j * W + il];
+        val = d.u_tmp[o] - scale * dp;
+      }
+      const bool ls = i > 0 ? d.solid[(size_t)j * W + i - 1] == 1 : false;
+      const bool rs = i < W ? d.solid[(size_t)j * W + i] == 1 : false;
+      d.u[o] = (ls || rs) ? 0.0 : val;
+    }
+    if (j <= H && i < W) {
+      const size_t o = (size_t)j * W + i;
+      double val = 0.0;
+      if (d.fmv[o]) {
+        const int jt = j < H ? j : H - 1;
+        const int jb2 = j - 1 < 0 ? H - 1 : j - 1;
+        const double dp = p[(size_t)jt * W + i] - p[(size_t)jb2 * W + i];
+        val = d.v_tmp[o] - scale * dp;
+      }
+      const bool ts = j > 0 ? d.solid[(size_t)(j - 1) * W + i] == 1 : false;
+      const bool bs = j < H ? d.solid[(size_t)j * W + i] == 1 : false;
+      d.v[o] = (ts || bs) ? 0.0 : val;
+    }
+  }
+}
+
+inline unsigned batch_grid(int njobs, int blocks) {
+  const int b = blocks > 0 ? blocks : 2048;
+  return (unsigned)(njobs < b ? (njobs > 0 ? njobs : 1) : b);
+}
+
+int batch_box_forces(const BatchScene* sc, const int4* jobs, int njobs, const int* parity,
+                     const BatchPhys& ph, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_box_forces, dim3(batch_grid(njobs, blocks)), dim3(BT_NT), 0, s, sc, jobs,
+                     njobs, parity, ph);
+  return check_launch("k_b_box_forces");
+}
+
+int batch_advect_velocity(const BatchScene* sc, const int4* jobs, int njobs, const BatchPhys& ph,
+                          int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_advect_velocity, dim3(batch_grid(njobs, blocks)), dim3(BT_NT), 0, s, sc,
+                     jobs, njobs, ph);
+  return check_launch("k_b_advect_velocity");
+}
+
+int batch_advect_density(const BatchScene* sc, const int4* jobs, int njobs, const int* parity,
+                         const BatchPhys& ph, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_advect_density, dim3(batch_grid(njobs, blocks)), dim3(BT_NT), 0, s, sc,
+                     jobs, njobs, parity, ph);
+  return check_launch("k_b_advect_density");
+}
+
+int batch_divergence_rhs_norm(const BatchScene* sc, const int4* cell_jobs, int njobs,
+                              double* partial, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_divergence_norm, dim3(batch_grid(njobs, blocks)), dim3(BT_NT), 0, s, sc,
+                     cell_jobs, njobs, partial);
+  return check_launch("k_b_divergence_norm");
+}
+
+int batch_gradient(const BatchScene* sc, const int4* jobs, int njobs, int* parity,
+                   const BatchPhys& ph, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_b_gradient, dim3(batch_grid(njobs, blocks)), dim3(BT_NT), 0, s, sc, jobs,
+                     njobs, parity, ph);
+  return check_launch("k_b_gradient");
 }
 
 }  // namespace smk

@@ -23,6 +23,7 @@
 // -ffp-contract=off: neighbour sum ((l + r) + d) + u in the storage type,
 // then (sum + alpha*div) * 1/4 and the SOR blend in float64 (Numba types the
 // Python-float coefficients as float64).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -105,7 +106,8 @@ __device__ __forceinline__ double residual_value(T ce, T nsum, T b, double alpha
 // in rows (2m, 2m+1) lane x updates the one cell of the pass's colour, so
 // every lane is busy and no index needs a division.
 template <typename T, int MODE, int RH, int NT>
-__device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a) {
+__device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int bx, int by,
+                                                   int part) {
   constexpr int RW = 64;
   constexpr int NW = NT / 64;
   __shared__ T sp[RH * RW];
@@ -118,7 +120,7 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a) {
   const int TX = RW - 2 * R, TY = RH - 2 * R;
   const int H = a.H, W = a.W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gx0 = blockIdx.x * TX - R, gy0 = blockIdx.y * TY - R;
+  const int gx0 = bx * TX - R, gy0 = by * TY - R;
   const int gx = gx0 + lane;
   const bool colin = gx >= 0 && gx < W;
 
@@ -239,7 +241,7 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a) {
     for (int cyb = 2 * wave; cyb < CY; cyb += 2 * NW) {
       const int cy = cyb + (lane >> 5);
       if (cy >= CY || cx >= CX) continue;
-      const int Jc = blockIdx.y * CY + cy, Ic = blockIdx.x * CX + cx;
+      const int Jc = by * CY + cy, Ic = bx * CX + cx;
       if (Jc >= Hc || Ic >= Wc) continue;
       // the four children's residuals, stored in the level's dtype
       // (solver.py:116-117) and summed in float64 over the fluid children
@@ -278,18 +280,20 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a) {
       acc += (xout && gy < H && (f & F_FLUID)) ? rv * rv : 0.0;
     }
     const double tot = block_sum<NT>(acc, red);
-    if (threadIdx.x == 0) a.norm_partial[blockIdx.y * gridDim.x + blockIdx.x] = tot;
+    if (threadIdx.x == 0) a.norm_partial[part] = tot;
   }
 }
 
 template <typename T, int MODE, int RH, int NT>
 __global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
-  smooth_tile64_body<T, MODE, RH, NT>(a);
+  smooth_tile64_body<T, MODE, RH, NT>(a, blockIdx.x, blockIdx.y,
+                                      blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 template <typename T, int RH, int NT>
 __global__ __launch_bounds__(NT) void k_smooth_fine(const SmoothArgs<T> a) {
-  smooth_tile64_body<T, MODE_RB, RH, NT>(a);
+  smooth_tile64_body<T, MODE_RB, RH, NT>(a, blockIdx.x, blockIdx.y,
+                                         blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Red-black smoothing with the region in registers.  A workgroup of NW
@@ -728,7 +732,7 @@ __device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t*
 }
 
 template <typename T, int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
+__device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   // Every level array is addressed as (LDS base + per-level offset): pointers
   // picked from an array at run time would lose the LDS address space and
@@ -867,6 +871,11 @@ __global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
 #undef SP
 #undef SB
 #undef SF
+}
+
+template <typename T, int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
+  vcycle_small_body<T, MODE, NT>(a);
 }
 
 constexpr int NB_X = 64, NB_Y = 4, NB_T = NB_X * NB_Y;
@@ -1927,3 +1936,6 @@ int smk_mg_solve(smk_mg* mg, void* p, const void* div, double rho, double h, dou
 }
 
 }  // extern "C"
+
+// batched scenes (multigrid side and step driver)
+#include "smk_batch.inl"
