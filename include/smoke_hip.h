@@ -203,6 +203,9 @@ typedef struct smk_mg_params {
   double tolerance;        /* > 0: stop at ||r||/||alpha*div|| <= tolerance */
   int precision;           /* SMK_PREC_*                                   */
   int inner_cycles;        /* mixed: FP32 V-cycles per refinement          */
+  double coarsest_omega;   /* coarsest level only: 0 = as smoother_omega
+                              (reference); > 0 forces it; < 0 the optimal
+                              SOR omega of the coarsest level's size (new) */
 } smk_mg_params;
 
 /* multigrid/solver.py:34-92 (_build_levels, _build_buffers, _ensure_cache):
@@ -322,6 +325,7 @@ typedef struct smk_batch_params {
   int smoother_iterations;     /* V(s,s), 1..3                            */
   int coarsest_iterations;
   int coarse_scale;            /* SMK_COARSE_*                            */
+  double coarsest_omega;       /* as smk_mg_params.coarsest_omega         */
   double tolerance;            /* relative residual per scene             */
   int max_cycles;              /* V-cycles per scene per step             */
   int use_graphs;              /* replay stages from HIP graphs           */

@@ -14,6 +14,8 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "lib", "libsmoke_hip.so"))
 
+ABI_VERSION = 2  # smk_abi_version() of the library these bindings describe
+
 SMK_F64 = 0
 SMK_F32 = 1
 
@@ -66,6 +68,7 @@ class MgParams(ctypes.Structure):
         ("tolerance", ctypes.c_double),
         ("precision", ctypes.c_int),
         ("inner_cycles", ctypes.c_int),
+        ("coarsest_omega", ctypes.c_double),
     ]
 
 
@@ -88,7 +91,7 @@ class BatchParams(ctypes.Structure):
         ("density_coefficient", ctypes.c_double), ("ambient_temperature", ctypes.c_double),
         ("fluid_density", ctypes.c_double), ("smoother_iterations", ctypes.c_int),
         ("coarsest_iterations", ctypes.c_int), ("coarse_scale", ctypes.c_int),
-        ("tolerance", ctypes.c_double), ("max_cycles", ctypes.c_int),
+        ("coarsest_omega", ctypes.c_double), ("tolerance", ctypes.c_double), ("max_cycles", ctypes.c_int),
         ("use_graphs", ctypes.c_int), ("tile_rows", ctypes.c_int),
         ("persistent_blocks", ctypes.c_int),
     ]
@@ -174,6 +177,9 @@ def load(path: str = LIB_PATH):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
+        if L.smk_abi_version() != ABI_VERSION:
+            raise HipError(f"{path} has ABI {L.smk_abi_version()}, these bindings expect "
+                           f"{ABI_VERSION}: rebuild it with python-game-engine_amd/build_native.py")
         _lib = L
     return _lib
 

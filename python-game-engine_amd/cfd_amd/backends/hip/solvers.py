@@ -157,7 +157,7 @@ class MultigridSolver:
                  dt, v_cycles=2, smoother_type="jacobi", smoother_backend="hip", num_levels=4,
                  smoother_iterations=2, coarsest_iterations=50, rebuild_masks=False,
                  profiler=None, smoother_omega=0.0, coarse_scale="reference", tolerance=0.0,
-                 precision="native", inner_cycles=1, **solver_kwargs):
+                 precision="native", inner_cycles=1, coarsest_omega=0.0, **solver_kwargs):
         _check(pressure, divergence, face_mask_u, face_mask_v)
         if smoother_type not in _SMOOTHERS:
             raise ValueError(
@@ -174,7 +174,7 @@ class MultigridSolver:
                        coarsest_iterations=int(coarsest_iterations),
                        smoother_omega=float(smoother_omega), coarse_scale=_COARSE[coarse_scale],
                        tolerance=float(tolerance), precision=_PREC[precision],
-                       inner_cycles=int(inner_cycles))
+                       inner_cycles=int(inner_cycles), coarsest_omega=float(coarsest_omega))
         info = SolveInfo()
         call("smk_mg_solve", self._handle.ptr, pressure.data_ptr(), divergence.data_ptr(),
              float(fluid_density), float(cell_size), float(dt), ctypes.byref(prm),

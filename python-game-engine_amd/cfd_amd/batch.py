@@ -38,7 +38,8 @@ class BatchedSmoke:
 
     def __init__(self, scenes, *, tolerance: float = 1e-6, max_cycles: int = 100,
                  smoother_iterations: int = 2, coarsest_iterations: int = 50,
-                 use_graphs: bool = True, tile_rows: int = 48, persistent_blocks: int = 0,
+                 coarsest_omega: float = -1.0, use_graphs: bool = True, tile_rows: int = 48,
+                 persistent_blocks: int = 0,
                  fluid_density: float = 1.0, device="cuda"):
         if not scenes:
             raise ValueError("need at least one scene")
@@ -95,7 +96,8 @@ class BatchedSmoke:
             density_coefficient=float(sc0.density_coefficient), ambient_temperature=0.0,
             fluid_density=float(fluid_density), smoother_iterations=int(smoother_iterations),
             coarsest_iterations=int(coarsest_iterations),
-            coarse_scale=_lib.SMK_COARSE_REDISCRETIZED, tolerance=float(tolerance),
+            coarse_scale=_lib.SMK_COARSE_REDISCRETIZED, coarsest_omega=float(coarsest_omega),
+            tolerance=float(tolerance),
             max_cycles=int(max_cycles), use_graphs=1 if use_graphs else 0,
             tile_rows=int(tile_rows), persistent_blocks=int(persistent_blocks))
         self._arr = arr

@@ -184,7 +184,8 @@ def uneven_scenes(count: int = 96, steps: int = 20, base_seed: int = 4000,
                          {"smoother_type": "rb_sor_gauss_seidel", "smoother_omega": 1.0,
                           "num_levels": mg_levels_to(min(H, W)), "smoother_iterations": 2,
                           "coarsest_iterations": 50, "v_cycles": 400,
-                          "coarse_scale": "rediscretized", "tolerance": 1e-6})
+                          "coarse_scale": "rediscretized", "tolerance": 1e-6,
+                          "coarsest_omega": -1.0})
         scenes.append(sc)
     return scenes
 

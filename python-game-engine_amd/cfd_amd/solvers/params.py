@@ -68,6 +68,7 @@ class MultigridParams:
     tolerance: float = 0.0  # new
     precision: str = "native"  # new
     inner_cycles: int = 1  # new
+    coarsest_omega: float = 0.0  # new: 0 = smoother_omega; < 0 optimal for the coarsest size
 
     def string(self):
         return (f"MultigridParams_smoother_{self.smoother_type}_{self.smoother_backend}"

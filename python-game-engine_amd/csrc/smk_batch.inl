@@ -299,6 +299,9 @@ static int batch_build_vcycle(smk_batch* b, hipStream_t s) {
       m.flags[k] = L.flags;
       m.alpha[k] = calc_alpha(hl, rho, dt);
       m.omega[k] = 1.0;
+      if (k == m.nl - 1 && b->prm.coarsest_omega != 0.0)
+        m.omega[k] = b->prm.coarsest_omega > 0.0 ? b->prm.coarsest_omega
+                                                 : optimal_omega(L.H > L.W ? L.H : L.W);
       m.coarse_div[k] = redisc ? m.alpha[k] : calc_alpha(hl * 2.0, rho, dt);
       hl *= 2.0;
     }

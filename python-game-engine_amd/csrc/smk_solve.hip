@@ -1377,14 +1377,18 @@ static int mg_alloc(void** ptr, size_t bytes) {
 struct MgRun {
   int mode;
   double omega_override;
+  double coarsest_omega;
   int si, ci;
   int rediscretize;
   double rho, dt;
 };
 
-inline double level_omega(const MgRun& run, const Level& L) {
+inline double level_omega(const MgRun& run, const Level& L, bool coarsest = false) {
   if (run.mode != MODE_RB) return 1.0;
-  return run.omega_override > 0.0 ? run.omega_override : optimal_omega(L.H > L.W ? L.H : L.W);
+  const int n = L.H > L.W ? L.H : L.W;
+  if (coarsest && run.coarsest_omega != 0.0)
+    return run.coarsest_omega > 0.0 ? run.coarsest_omega : optimal_omega(n);
+  return run.omega_override > 0.0 ? run.omega_override : optimal_omega(n);
 }
 
 // First level from which the rest of the hierarchy fits one workgroup's LDS.
@@ -1481,7 +1485,7 @@ int mg_small(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zer
     a.W[k] = L.W;
     a.flags[k] = L.flags;
     a.alpha[k] = calc_alpha(hl, run.rho, run.dt);
-    a.omega[k] = level_omega(run, L);
+    a.omega[k] = level_omega(run, L, k == a.nl - 1);
     a.coarse_div[k] = run.rediscretize ? a.alpha[k] : calc_alpha(hl * 2.0, run.rho, run.dt);
     hl *= 2.0;
   }
@@ -1516,7 +1520,7 @@ int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool ze
   const int n_iter = coarsest ? run.ci : run.si;
   const int npass = run.mode == MODE_RB ? 2 * n_iter : n_iter;
   const double alpha = calc_alpha(h, run.rho, run.dt);
-  const double omega = level_omega(run, L);
+  const double omega = level_omega(run, L, coarsest);
   if (coarsest)
     return mg_smooth<T>(lv, l, use32, npass, zero_init, nullptr, 0, 0, nullptr, 1.0,
                         norm_partial, nparts, alpha, omega, run.mode, s);
@@ -1537,6 +1541,7 @@ inline MgRun make_run(const smk_mg_params* prm, double rho, double dt) {
   MgRun run{};
   run.mode = prm->smoother == SMK_SMOOTH_JACOBI ? MODE_JACOBI : MODE_RB;
   run.omega_override = prm->smoother_omega;
+  run.coarsest_omega = prm->coarsest_omega;
   run.si = prm->smoother_iterations;
   run.ci = prm->coarsest_iterations;
   run.rediscretize = prm->coarse_scale == SMK_COARSE_REDISCRETIZED;
