@@ -154,20 +154,138 @@ def cpu_baseline(scene, sample_steps: int):
                        f"{cyc / sample_steps:.1f} V-cycles/step, {dt:.1f} s")}
 
 
+def bench_batched(count: int, steps: int) -> dict:
+    """The uneven-scenes workload: ``count`` scenes (N log-uniform in
+    [64, 1024], multiple of 16, aspect 1:1 or 2:1, seeds 4000+s), one
+    batch, FP32 V-cycles in FP64 defect correction to 1e-6 per scene; the
+    autotuner picks tile height, sweeps and grid size first."""
+    import torch
+
+    from cfd_amd import scenes
+    from cfd_amd.batch import BatchedSmoke
+
+    sl = scenes.uneven_scenes(count=count, steps=steps)
+    b = BatchedSmoke(sl, tolerance=1e-6)
+    b.step()
+    tune = b.autotune(tile_rows=(48, 32), smoother_iterations=(1, 2, 3),
+                      persistent_blocks=(2048, 4096), repeats=1)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    infos = []
+    e0.record()
+    for _ in range(steps):
+        infos.append(b.step())
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    cyc, rel = b.scene_stats()
+    out = {"workload": f"uneven_{count}", "scenes": count, "cells": b.cells, "steps": steps,
+           "ms_per_step": ms, "cell_steps_per_s": b.cells / (ms / 1e3),
+           "tuning": tune["chosen"], "iterations_per_step": [i["iterations"] for i in infos],
+           "v_cycles_last_step": sum(cyc), "max_rel_residual": max(rel)}
+    b.close()
+    # the same scenes one at a time (single-scene path), a few steps
+    sims = [scenes.make_simulation(s, dtype="float64", device="cuda",
+                                   solver_overrides={"precision": "mixed", "inner_cycles": 1,
+                                                     "smoother_iterations":
+                                                         tune["chosen"]["smoother_iterations"]})
+            for s in sl]
+    for sim, s in zip(sims, sl):
+        sim.compute_metrics = False
+        sim.substep(s.dt)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(2):
+        for sim, s in zip(sims, sl):
+            sim.substep(s.dt)
+    e1.record()
+    torch.cuda.synchronize()
+    out["one_at_a_time_ms_per_step"] = e0.elapsed_time(e1) / 2
+    out["speedup_vs_one_at_a_time"] = out["one_at_a_time_ms_per_step"] / ms
+    return out
+
+
+def bench_differentiable(steps: int, batch: int) -> dict:
+    """The 256^2 differentiable rollout: 64 steps, MG to 1e-8, controls u0,
+    v0 (zero) and per-step strengths (1.0), loss 0.5||rho_T - target||^2
+    with a Gaussian target; forward-only vs forward + adjoint (checkpoint
+    every 8 steps, recompute between) in FP32 and FP64, and a batch of
+    seeded targets."""
+    import torch
+
+    from cfd_amd import diff, scenes
+
+    sc = scenes.diff_rollout_256(steps=steps)
+    N = sc.H
+    out = {"workload": f"diff_{N}", "steps": steps}
+
+    def ev():
+        return torch.cuda.Event(enable_timing=True)
+
+    for name, dt_ in (("fp32", torch.float32), ("fp64", torch.float64)):
+        m = diff.from_scene(sc, dtype=dt_)
+        tgt = torch.as_tensor(diff.gaussian_target(N, N, sc.cell_size), dtype=dt_, device="cuda")
+
+        def controls(grad):
+            z = dict(dtype=dt_, device="cuda", requires_grad=grad)
+            return (torch.zeros((N, N + 1), **z), torch.zeros((N + 1, N), **z),
+                    torch.ones(steps, **z))
+
+        def fwd():
+            u0, v0, s = controls(False)
+            return m.forward_only(u0, v0, s, steps)
+
+        def fb(target=tgt):
+            u0, v0, s = controls(True)
+            rho, _, _ = m.rollout(u0, v0, s, steps, checkpoint_every=8)
+            diff.density_loss(rho, target).backward()
+            return u0.grad, v0.grad, s.grad
+
+        fwd()
+        fb()
+        res = {}
+        for key, f in (("forward_ms", fwd), ("forward_adjoint_ms", fb)):
+            a, c = ev(), ev()
+            a.record()
+            f()
+            c.record()
+            torch.cuda.synchronize()
+            res[key] = a.elapsed_time(c)
+        res["grad_fwd_ratio"] = res["forward_adjoint_ms"] / res["forward_ms"]
+        if name == "fp32" and batch > 0:
+            targets = [torch.as_tensor(t, dtype=dt_, device="cuda")
+                       for t in diff.seeded_targets(N, N, sc.cell_size, range(500, 500 + batch))]
+            a, c = ev(), ev()
+            a.record()
+            for t in targets:
+                fb(t)
+            c.record()
+            torch.cuda.synchronize()
+            res["batch_targets"] = batch
+            res["batch_ms"] = a.elapsed_time(c)
+        out[name] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--N", type=int, default=2048)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", default="fp64,mixed")
+    ap.add_argument("--batched-scenes", type=int, default=96,
+                    help="scenes of the uneven-batch workload (0 skips it)")
+    ap.add_argument("--diff-steps", type=int, default=64,
+                    help="steps of the differentiable rollout (0 skips it)")
+    ap.add_argument("--diff-batch", type=int, default=16)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(it measures one GPU; nothing here runs across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -206,6 +324,16 @@ def main():
         cpu = cpu_baseline(scene, args.cpu_steps)
         log(f"cpu baseline: {cpu}")
 
+    extra = {}
+    if args.batched_scenes > 0:
+        log(f"batched scenes: {args.batched_scenes}")
+        extra["batched_scenes"] = bench_batched(args.batched_scenes, 20)
+        log(f"batched scenes: {json.dumps(extra['batched_scenes'])}")
+    if args.diff_steps > 0:
+        log(f"differentiable rollout: {args.diff_steps} steps")
+        extra["differentiable"] = bench_differentiable(args.diff_steps, args.diff_batch)
+        log(f"differentiable rollout: {json.dumps(extra['differentiable'])}")
+
     line = {
         "metric": "smoke cell-steps/s & %HBM roofline at 2048² MG (mixed vs FP64); grad/fwd ratio",
         "value": m["cell_steps_per_s"],
@@ -235,6 +363,9 @@ def main():
     }
     if "fp64" in results and "mixed" in results:
         line["mixed_speedup_vs_fp64"] = results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"]
+    if "differentiable" in extra:
+        line["grad_fwd_ratio"] = extra["differentiable"]["fp32"]["grad_fwd_ratio"]
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

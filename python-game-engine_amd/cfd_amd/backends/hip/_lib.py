@@ -224,8 +224,12 @@ def profile_read(reset: bool = True) -> list:
 
 
 SMOOTHER_VARIANTS = {"lds48": 0, "lds32": 1, "reg64": 2}
+SMALL_VARIANTS = {"nt256": 0, "nt512": 1, "nt1024": 2}
+_WHICH = {"fine": 0, "coarse": 1, "small": 2}
 
 
 def set_smoother_variant(dtype_code: int, which: str, variant: str) -> None:
-    call("smk_set_smoother_variant", int(dtype_code), 0 if which == "fine" else 1,
-         SMOOTHER_VARIANTS[variant])
+    """which: "fine" / "coarse" (tile variants lds48, lds32, reg64) or
+    "small" (small-level workgroup size nt256, nt512, nt1024)."""
+    table = SMALL_VARIANTS if which == "small" else SMOOTHER_VARIANTS
+    call("smk_set_smoother_variant", int(dtype_code), _WHICH[which], table[variant])

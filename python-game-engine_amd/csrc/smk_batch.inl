@@ -165,7 +165,8 @@ struct smk_batch {
   int* d_cycles = nullptr;
   int* d_count = nullptr;
   int* d_parity = nullptr;
-  int* h_count = nullptr;  // pinned
+  int* h_count = nullptr;  // pinned ring, one slot per queued iteration
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // V-cycle
   std::vector<smk::BatchDepth> depth;
   smk::SmoothArgs<float>* d_args = nullptr;
@@ -433,6 +434,8 @@ static void batch_release(smk_batch* b) {
                   (void*)b->d_count, (void*)b->d_parity})
     if (p) (void)hipFree(p);
   if (b->h_count) (void)hipHostFree(b->h_count);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
   for (smk_mg* m : b->mg)
     if (m) smk_mg_destroy(m);
   if (b->cap) (void)hipStreamDestroy(b->cap);
@@ -539,8 +542,10 @@ int smk_batch_create(const smk_batch_scene* scenes, int n, const smk_batch_param
   if (!rc) rc = mg_alloc((void**)&b->d_count, sizeof(int));
   if (!rc) rc = mg_alloc((void**)&b->d_parity, sizeof(int));
   if (!rc) {
-    hipError_t e = hipHostMalloc((void**)&b->h_count, sizeof(int), hipHostMallocDefault);
-    if (e != hipSuccess) rc = fail((int)e, "hipHostMalloc failed: %s", hipGetErrorString(e));
+    hipError_t e = hipHostMalloc((void**)&b->h_count, 4 * sizeof(int), hipHostMallocDefault);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
+      e = hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) rc = fail((int)e, "smk_batch_create: %s", hipGetErrorString(e));
   }
   if (!rc) {
     hipError_t e = hipMemsetAsync(b->d_parity, 0, sizeof(int), s);
@@ -574,7 +579,10 @@ int smk_batch_step(smk_batch* b, smk_batch_info* info, void* stream) {
     rc = batch_stage_pre(b, s);
     if (rc) return rc;
   }
-  int it = 0;
+  // the active count of iteration k-1 is read while iteration k is queued
+  // (an iteration after the last useful one is a no-op: every launch skips
+  // inactive scenes)
+  int it = 0, done_at = -1;
   for (; it <= b->prm.max_cycles + 1; ++it) {
     if (graphs) {
       SMK_CHECK_HIP(hipGraphLaunch(b->g_iter, s));
@@ -582,10 +590,20 @@ int smk_batch_step(smk_batch* b, smk_batch_info* info, void* stream) {
       rc = batch_stage_iter(b, s);
       if (rc) return rc;
     }
-    SMK_CHECK_HIP(hipMemcpyAsync(b->h_count, b->d_count, sizeof(int), hipMemcpyDeviceToHost, s));
-    SMK_CHECK_HIP(hipStreamSynchronize(s));
-    if (*b->h_count == 0) break;
+    const int slot = it & 3;
+    SMK_CHECK_HIP(hipMemcpyAsync(&b->h_count[slot], b->d_count, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    SMK_CHECK_HIP(hipEventRecord(b->ev[slot], s));
+    if (it >= 1) {
+      const int prev = (it - 1) & 3;
+      SMK_CHECK_HIP(hipEventSynchronize(b->ev[prev]));
+      if (b->h_count[prev] == 0) {
+        done_at = it - 1;
+        break;
+      }
+    }
   }
+  if (done_at >= 0) it = done_at;
   if (graphs) {
     SMK_CHECK_HIP(hipGraphLaunch(b->g_post, s));
   } else {

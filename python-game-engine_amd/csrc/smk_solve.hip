@@ -55,6 +55,8 @@ struct SmoothArgs {
   double coarse_div;
   // norm-out
   double* norm_partial;
+  // the launch does nothing while *gate == 0 (device-side convergence)
+  const int* gate;
 };
 
 template <typename T>
@@ -116,6 +118,7 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
   __shared__ uint8_t sf[RH * RW];
   __shared__ double red[NW];
 
+  if (a.gate != nullptr && *a.gate == 0) return;
   const int R = a.R;
   const int TX = RW - 2 * R, TY = RH - 2 * R;
   const int H = a.H, W = a.W;
@@ -340,6 +343,7 @@ __global__ __launch_bounds__(NW * 64) void k_rb_reg(const SmoothArgs<T> a) {
   static_assert(BR % 2 == 0, "bands hold whole row pairs");
   __shared__ T edge[2][NW][2][64];
   __shared__ double red[NW];
+  if (a.gate != nullptr && *a.gate == 0) return;
 
   const int R = a.R;
   const int TX = RW - 2 * R, TY = RH - 2 * R;
@@ -543,6 +547,7 @@ struct SmallArgs {
   int zero_top;
   int npass_smooth;    // per pre/post smoothing (RB: 2*sweeps)
   int npass_coarsest;  // coarsest level
+  const int* gate;     // nothing is done while *gate == 0
 };
 
 // One red-black colour pass or Jacobi iteration over a whole level in LDS.
@@ -734,6 +739,7 @@ __device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t*
 template <typename T, int MODE, int NT>
 __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
+  if (a.gate != nullptr && *a.gate == 0) return;
   // Every level array is addressed as (LDS base + per-level offset): pointers
   // picked from an array at run time would lose the LDS address space and
   // turn every access into a FLAT one.  Layout: [p_l, b_l]* tmp | flags_l*.
@@ -889,8 +895,10 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
                                                         const uint8_t* __restrict__ flags,
                                                         int H, int W, double alpha,
                                                         double* __restrict__ partial,
-                                                        float* __restrict__ defect) {
+                                                        float* __restrict__ defect,
+                                                        const int* __restrict__ gate) {
   __shared__ double red[NB_T / 64];
+  if (gate != nullptr && *gate == 0) return;
   const int i = blockIdx.x * NB_X + (threadIdx.x & (NB_X - 1));
   const int j = blockIdx.y * NB_Y + (threadIdx.x / NB_X);
   double acc = 0.0;
@@ -945,9 +953,51 @@ __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict_
 }
 
 template <typename T>
-__global__ void k_add_correction(T* __restrict__ p, const float* __restrict__ e, long n) {
+__global__ void k_add_correction(T* __restrict__ p, const float* __restrict__ e, long n,
+                                 const int* __restrict__ gate) {
+  if (gate != nullptr && *gate == 0) return;
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = p[k] + (T)e[k];
+}
+
+// Convergence state of a tolerance-driven solve, kept on the device so that
+// the host can queue the next iteration before it knows the outcome of the
+// previous one: every launch of an iteration is gated on `active`.
+struct SolveState {
+  int active;   // 1 while the solve continues
+  int cycles;   // refinements (mixed) or V-cycles (native) queued to run
+  double rel;   // relative residual of the last check
+};
+
+__global__ void k_state_init(SolveState* __restrict__ st) {
+  st->active = 1;
+  st->cycles = 0;
+  st->rel = -1.0;
+}
+
+// One convergence check: the k_sum_partials reduction of ||r||^2, the
+// relative residual against ||alpha div||^2 = *b2, and the decision the host
+// loops of the solvers take (stop at rel <= tol, or after `cap` cycles --
+// `inner` V-cycles per check).
+__global__ __launch_bounds__(1024) void k_check(const double* __restrict__ partial, int n,
+                                                const double* __restrict__ b2,
+                                                SolveState* __restrict__ st, double tol,
+                                                int cap, int inner) {
+  __shared__ double red[1024 / 64];
+  if (st->active == 0) return;
+  double acc = 0.0;
+  for (int k = threadIdx.x; k < n; k += 1024) acc += partial[k];
+  const double tot = block_sum<1024>(acc, red);
+  if (threadIdx.x != 0) return;
+  const double bb = *b2;
+  const double rel = bb > 0.0 ? sqrt(tot / bb) : 0.0;
+  st->rel = rel;
+  const int c = st->cycles;
+  const bool done = (tol > 0.0 ? rel <= tol : c >= cap) || c * inner >= cap;
+  if (done)
+    st->active = 0;
+  else
+    st->cycles = c + 1;
 }
 
 __global__ void k_solid_from_flags(const uint8_t* __restrict__ flags, long n,
@@ -1015,6 +1065,10 @@ constexpr int JAC_RH = 32;
 // level gets its own kernel symbol so profiles separate the two.
 enum : int { VAR_LDS48 = 0, VAR_LDS32 = 1, VAR_REG64 = 2, NUM_VARIANTS = 3 };
 static int g_variant[2][2] = {{VAR_LDS48, VAR_LDS32}, {VAR_LDS48, VAR_LDS32}};  // [f32?][coarse?]
+
+// Threads of the single-workgroup small-level V-cycle, per dtype [f32?].
+static int g_small_nt[2] = {512, 512};
+constexpr int SMALL_NT_CHOICES[3] = {256, 512, 1024};
 
 inline int variant_rh(int v) { return v == VAR_LDS48 ? 48 : (v == VAR_LDS32 ? 32 : RB_RH); }
 
@@ -1094,13 +1148,18 @@ static size_t small_lds_limit() {
   if (want > 160 * 1024) want = 160 * 1024;
   if (want < 64 * 1024) want = 64 * 1024;
   if (want > 64 * 1024) {
-    hipError_t e1 = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, SMALL_NT>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)want);
-    hipError_t e2 = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, SMALL_NT>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)want);
-    if (e1 != hipSuccess || e2 != hipSuccess) {
+    const void* fns[] = {
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 256>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, 256>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 512>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, 512>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 1024>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, 1024>)};
+    bool ok = true;
+    for (const void* f : fns)
+      ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want) ==
+                     hipSuccess;
+    if (!ok) {
       (void)hipGetLastError();
       want = 64 * 1024;
     }
@@ -1130,12 +1189,21 @@ int launch_small(int mode, const SmallArgs<T>& a, size_t lds, hipStream_t s) {
     hipStream_t s;
     ~Done() { prof_end(ev, sizeof(T) == 4 ? "vcycle_small_f32" : "vcycle_small_f64", bytes, s); }
   } done{ev, bytes, s};
-  if (mode == MODE_RB)
-    hipLaunchKernelGGL((k_vcycle_small<T, MODE_RB, SMALL_NT>), dim3(1), dim3(SMALL_NT), lds, s,
-                       a);
+  const int nt = g_small_nt[sizeof(T) == 4];
+#define SMK_SMALL_LAUNCH(NTV)                                                                   \
+  do {                                                                                         \
+    if (mode == MODE_RB)                                                                       \
+      hipLaunchKernelGGL((k_vcycle_small<T, MODE_RB, NTV>), dim3(1), dim3(NTV), lds, s, a);    \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_vcycle_small<T, MODE_JACOBI, NTV>), dim3(1), dim3(NTV), lds, s, a); \
+  } while (0)
+  if (nt == 1024)
+    SMK_SMALL_LAUNCH(1024);
+  else if (nt == 512)
+    SMK_SMALL_LAUNCH(512);
   else
-    hipLaunchKernelGGL((k_vcycle_small<T, MODE_JACOBI, SMALL_NT>), dim3(1), dim3(SMALL_NT), lds,
-                       s, a);
+    SMK_SMALL_LAUNCH(256);
+#undef SMK_SMALL_LAUNCH
   return check_launch("k_vcycle_small");
 }
 
@@ -1252,11 +1320,11 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
 template <typename T>
 int residual_norm2_dev(const T* p, const T* rhs, const uint8_t* flags, int H, int W,
                        double alpha, double* partial, double* result, int slot, float* defect,
-                       hipStream_t s) {
+                       hipStream_t s, const int* gate = nullptr) {
   dim3 g(cdiv(W, NB_X), cdiv(H, NB_Y));
   hipEvent_t ev = prof_begin(s);
   hipLaunchKernelGGL(k_residual_norm<T>, g, dim3(NB_T), 0, s, p, rhs, flags, H, W, alpha,
-                     partial, defect);
+                     partial, defect, gate);
   prof_end(ev, defect ? "defect_fp64" : "residual_norm",
            (2.0 * sizeof(T) + (defect ? 4.0 : 0.0)) * (double)H * W, s);
   int rc = check_launch("k_residual_norm");
@@ -1264,6 +1332,20 @@ int residual_norm2_dev(const T* p, const T* rhs, const uint8_t* flags, int H, in
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partial, (int)(g.x * g.y),
                      result, slot);
   return check_launch("k_sum_partials");
+}
+
+// k_residual_norm only: the partials for a k_check of the same launch shape.
+template <typename T>
+int residual_partials_dev(const T* p, const T* rhs, const uint8_t* flags, int H, int W,
+                          double alpha, double* partial, float* defect, const int* gate,
+                          hipStream_t s) {
+  dim3 g(cdiv(W, NB_X), cdiv(H, NB_Y));
+  hipEvent_t ev = prof_begin(s);
+  hipLaunchKernelGGL(k_residual_norm<T>, g, dim3(NB_T), 0, s, p, rhs, flags, H, W, alpha,
+                     partial, defect, gate);
+  prof_end(ev, defect ? "defect_fp64" : "residual_norm",
+           (2.0 * sizeof(T) + (defect ? 4.0 : 0.0)) * (double)H * W, s);
+  return check_launch("k_residual_norm");
 }
 
 template <typename T>
@@ -1362,6 +1444,11 @@ struct smk_mg {
   double* partial;
   double* result;
   int npartial;
+  // device-side convergence (gated iterations); host copies land in a
+  // pinned ring, one slot per queued iteration
+  smk::SolveState* d_state;
+  smk::SolveState* h_ring;
+  hipEvent_t ev[4];
   int small_from[2];  // first level handled by k_vcycle_small: [native dtype, float32]
   int small_mode[2];  // smoother mode small_from was computed for
 };
@@ -1378,6 +1465,7 @@ struct MgRun {
   int mode;
   double omega_override;
   double coarsest_omega;
+  const int* gate;  // device flag gating every launch (nullptr: ungated)
   int si, ci;
   int rediscretize;
   double rho, dt;
@@ -1414,7 +1502,7 @@ template <typename T>
 int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_init,
               const T* coarse_e, int Hc, int Wc, T* coarse_rhs, double coarse_div,
               double* norm_partial, int* nparts, double alpha, double omega, int mode,
-              hipStream_t s) {
+              hipStream_t s, const int* gate = nullptr) {
   Level& L = lv[l];
   void** pb = use32 ? L.p32 : L.p;
   int& cur = use32 ? L.cur32 : L.cur;
@@ -1449,6 +1537,7 @@ int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_in
         g_variant[sizeof(T) == 4][l == 0 ? 0 : 1] == VAR_REG64)
       a.R += a.R & 1;
     a.zero_init = (first && zero_init) ? 1 : 0;
+    a.gate = gate;
     if (first && coarse_e != nullptr) {
       a.coarse_e = coarse_e;
       a.Hc = Hc;
@@ -1498,6 +1587,7 @@ int mg_small(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zer
   a.zero_top = zero_init ? 1 : 0;
   a.npass_smooth = run.mode == MODE_RB ? 2 * run.si : run.si;
   a.npass_coarsest = run.mode == MODE_RB ? 2 * run.ci : run.ci;
+  a.gate = run.gate;
   const size_t lds = small_lds_bytes(a.H, a.W, a.nl, sizeof(T), run.mode);
   return launch_small<T>(run.mode, a, lds, s);
 }
@@ -1523,18 +1613,18 @@ int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool ze
   const double omega = level_omega(run, L, coarsest);
   if (coarsest)
     return mg_smooth<T>(lv, l, use32, npass, zero_init, nullptr, 0, 0, nullptr, 1.0,
-                        norm_partial, nparts, alpha, omega, run.mode, s);
+                        norm_partial, nparts, alpha, omega, run.mode, s, run.gate);
   Level& C = lv[l + 1];
   T* crhs = reinterpret_cast<T*>(use32 ? C.rhs32 : C.rhs);
   const double coarse_div = run.rediscretize ? alpha : calc_alpha(h * 2.0, run.rho, run.dt);
   int rc = mg_smooth<T>(lv, l, use32, npass, zero_init, nullptr, 0, 0, crhs, coarse_div, nullptr,
-                        nullptr, alpha, omega, run.mode, s);
+                        nullptr, alpha, omega, run.mode, s, run.gate);
   if (rc) return rc;
   rc = mg_vcycle<T>(mg, l + 1, use32, run, h * 2.0, true, nullptr, nullptr, s);
   if (rc) return rc;
   const T* ce = reinterpret_cast<const T*>(use32 ? C.p32[C.cur32] : C.p[C.cur]);
   return mg_smooth<T>(lv, l, use32, npass, false, ce, C.H, C.W, nullptr, 1.0, norm_partial,
-                      nparts, alpha, omega, run.mode, s);
+                      nparts, alpha, omega, run.mode, s, run.gate);
 }
 
 inline MgRun make_run(const smk_mg_params* prm, double rho, double dt) {
@@ -1577,6 +1667,42 @@ int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, do
   return SMK_OK;
 }
 
+// Queue gated iterations -- iter(k) enqueues iteration k, ending with a
+// k_check on mg->d_state -- until the device reports the solve done.  The
+// outcome of iteration k-1 is read (from a pinned ring slot) while iteration
+// k is already queued, so the GPU never waits for the host; the one
+// iteration queued after the last useful one is gated off and does nothing.
+template <typename F>
+int run_gated(smk_mg* mg, int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
+  for (int k = 0; k <= max_iter; ++k) {
+    int rc = iter(k);
+    if (rc) return rc;
+    const int slot = k & 3;
+    SMK_CHECK_HIP(hipMemcpyAsync(&mg->h_ring[slot], mg->d_state, sizeof(SolveState),
+                                 hipMemcpyDeviceToHost, s));
+    SMK_CHECK_HIP(hipEventRecord(mg->ev[slot], s));
+    if (k >= 1) {
+      const int prev = (k - 1) & 3;
+      SMK_CHECK_HIP(hipEventSynchronize(mg->ev[prev]));
+      if (mg->h_ring[prev].active == 0) break;
+    }
+  }
+  SMK_CHECK_HIP(hipStreamSynchronize(s));
+  SMK_CHECK_HIP(hipMemcpy(fin, mg->d_state, sizeof(SolveState), hipMemcpyDeviceToHost));
+  return SMK_OK;
+}
+
+// Gated iterations need the finest level's ping-pong flips per V-cycle to be
+// even (one launch per pre- and post-smoothing, or none when the small-level
+// kernel owns level 0), so that skipped V-cycles leave the host's record of
+// which buffer holds the result where executed ones would.  Coarser levels
+// restart from zero every cycle, so their buffer parity does not matter.
+inline bool gateable(const smk_mg* mg, const MgRun& run, int slot) {
+  if (mg->small_from[slot] == 0) return true;
+  const int npass = run.mode == MODE_RB ? 2 * run.si : run.si;
+  return mg->nl > 1 && npass + 1 <= MAX_R;
+}
+
 template <typename T>
 int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double dt,
                     const smk_mg_params* prm, smk_solve_info* info, hipStream_t s) {
@@ -1591,7 +1717,35 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
   int cycles = 0;
   double rel = -1.0;
   bool converged = false;
-  if (prm->tolerance > 0.0) {
+  if (prm->tolerance > 0.0 && gateable(mg, run, 0)) {
+    // norm of the initial guess, then V-cycle + norm per gated iteration
+    rc = rhs_norm2_dev<T>(div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 1, s);
+    if (rc) return rc;
+    MgRun grun = run;
+    grun.gate = &mg->d_state->active;
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, s, mg->d_state);
+    rc = check_launch("k_state_init");
+    if (rc) return rc;
+    SolveState fin{};
+    rc = run_gated(mg, prm->v_cycles + 1, [&](int k) -> int {
+      int nparts = -1;
+      int r = SMK_OK;
+      if (k > 0) r = mg_vcycle<T>(mg, 0, false, grun, h, false, mg->partial, &nparts, s);
+      if (!r && nparts < 0) {
+        nparts = mg->npartial;
+        r = residual_partials_dev<T>(reinterpret_cast<const T*>(L0.p[L0.cur]), div, L0.flags,
+                                     mg->H, mg->W, alpha, mg->partial, nullptr, grun.gate, s);
+      }
+      if (r) return r;
+      hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
+                         mg->result + 1, mg->d_state, prm->tolerance, prm->v_cycles, 1);
+      return check_launch("k_check");
+    }, &fin, s);
+    if (rc) return rc;
+    cycles = fin.cycles;
+    rel = fin.rel;
+    converged = rel <= prm->tolerance;
+  } else if (prm->tolerance > 0.0) {
     rc = rhs_norm2_dev<T>(div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 1, s);
     if (rc) return rc;
     double host[2];
@@ -1666,7 +1820,43 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
   int refinements = 0, inner_total = 0;
   double rel = -1.0, b2 = 0.0;
   double host[2];
-  for (;;) {
+  if (gateable(mg, run, 1)) {
+    // iteration 0: defect of the initial p + check; iteration k > 0: inner
+    // FP32 V-cycles, p += e, defect + check -- all gated on the device state
+    MgRun grun = run;
+    grun.gate = &mg->d_state->active;
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, s, mg->d_state);
+    rc = check_launch("k_state_init");
+    if (rc) return rc;
+    const int nparts = norm_blocks(H, W);
+    SolveState fin{};
+    rc = run_gated(mg, prm->v_cycles + 1, [&](int k) -> int {
+      int r = SMK_OK;
+      if (k > 0) {
+        for (int c = 0; c < inner && !r; ++c)
+          r = mg_vcycle<float>(mg, 0, true, grun, h, c == 0, nullptr, nullptr, s);
+        if (r) return r;
+        const float* e = reinterpret_cast<const float*>(L0.p32[L0.cur32]);
+        hipEvent_t ev = prof_begin(s);
+        hipLaunchKernelGGL(k_add_correction<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, e,
+                           n, grun.gate);
+        prof_end(ev, "correct_fp64", 20.0 * (double)n, s);
+        r = check_launch("k_add_correction");
+        if (r) return r;
+      }
+      r = residual_partials_dev<double>(p, div, L0.flags, H, W, alpha, mg->partial,
+                                        reinterpret_cast<float*>(L0.rhs32), grun.gate, s);
+      if (r) return r;
+      hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
+                         mg->result + 1, mg->d_state, tol, prm->v_cycles, inner);
+      return check_launch("k_check");
+    }, &fin, s);
+    if (rc) return rc;
+    refinements = fin.cycles;
+    inner_total = fin.cycles * inner;
+    rel = fin.rel;
+  }
+  while (!gateable(mg, run, 1)) {
     // defect in FP32 into level-0 rhs32, norm^2 of r into result[0]
     rc = residual_norm2_dev<double>(p, div, L0.flags, H, W, alpha, mg->partial, mg->result, 0,
                                     reinterpret_cast<float*>(L0.rhs32), s);
@@ -1684,7 +1874,8 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     }
     const float* e = reinterpret_cast<const float*>(L0.p32[L0.cur32]);
     hipEvent_t ev = prof_begin(s);
-    hipLaunchKernelGGL(k_add_correction<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, e, n);
+    hipLaunchKernelGGL(k_add_correction<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, e, n,
+                       nullptr);
     prof_end(ev, "correct_fp64", 20.0 * (double)n, s);
     rc = check_launch("k_add_correction");
     if (rc) return rc;
@@ -1744,7 +1935,13 @@ int smk_profile_read(smk_profile_record* out, int n, int reset) {
 
 int smk_set_smoother_variant(int dtype, int which, int variant) {
   SMK_REQUIRE(dtype == SMK_F64 || dtype == SMK_F32, "smk_set_smoother_variant: bad dtype");
-  SMK_REQUIRE(which == 0 || which == 1, "smk_set_smoother_variant: which is 0 (fine) or 1 (coarse)");
+  SMK_REQUIRE(which >= 0 && which <= 2,
+              "smk_set_smoother_variant: which is 0 (fine), 1 (coarse) or 2 (small levels)");
+  if (which == 2) {
+    SMK_REQUIRE(variant >= 0 && variant < 3, "smk_set_smoother_variant: bad variant %d", variant);
+    g_small_nt[dtype == SMK_F32] = SMALL_NT_CHOICES[variant];
+    return SMK_OK;
+  }
   SMK_REQUIRE(variant >= 0 && variant < NUM_VARIANTS, "smk_set_smoother_variant: bad variant %d",
               variant);
   g_variant[dtype == SMK_F32][which] = variant;
@@ -1752,7 +1949,11 @@ int smk_set_smoother_variant(int dtype, int which, int variant) {
 }
 
 int smk_get_smoother_variant(int dtype, int which) {
-  if ((dtype != SMK_F64 && dtype != SMK_F32) || (which != 0 && which != 1)) return -1;
+  if ((dtype != SMK_F64 && dtype != SMK_F32) || which < 0 || which > 2) return -1;
+  if (which == 2) {
+    const int nt = g_small_nt[dtype == SMK_F32];
+    return nt == 1024 ? 2 : (nt == 512 ? 1 : 0);
+  }
   return g_variant[dtype == SMK_F32][which];
 }
 
@@ -1842,6 +2043,10 @@ int smk_mg_destroy(smk_mg* mg) {
   }
   (void)hipFree(mg->partial);
   (void)hipFree(mg->result);
+  (void)hipFree(mg->d_state);
+  if (mg->h_ring) (void)hipHostFree(mg->h_ring);
+  for (hipEvent_t e : mg->ev)
+    if (e) (void)hipEventDestroy(e);
   delete mg;
   return SMK_OK;
 }
@@ -1889,6 +2094,13 @@ int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W, int num_
   mg->npartial = norm_blocks(H, W);
   if (!rc) rc = mg_alloc((void**)&mg->partial, (size_t)mg->npartial * sizeof(double));
   if (!rc) rc = mg_alloc((void**)&mg->result, 4 * sizeof(double));
+  if (!rc) rc = mg_alloc((void**)&mg->d_state, sizeof(SolveState));
+  if (!rc) {
+    hipError_t e = hipHostMalloc((void**)&mg->h_ring, 4 * sizeof(SolveState), hipHostMallocDefault);
+    for (int k = 0; k < 4 && e == hipSuccess; ++k)
+      e = hipEventCreateWithFlags(&mg->ev[k], hipEventDisableTiming);
+    if (e != hipSuccess) rc = fail((int)e, "smk_mg_create: %s", hipGetErrorString(e));
+  }
   // level 0 flags from the caller's masks; solid = ~is_fluid (solver.py:38-40)
   if (!rc) rc = smk_cell_flags(fmu, fmv, H, W, mg->lv[0].flags, stream);
   if (!rc) {
