@@ -281,6 +281,8 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=64,
                     help="steps of the differentiable rollout (0 skips it)")
     ap.add_argument("--diff-batch", type=int, default=16)
+    ap.add_argument("--only", choices=("batched", "diff"), default=None,
+                    help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -295,6 +297,13 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+
+    if args.only == "batched":
+        print(json.dumps(bench_batched(args.batched_scenes or 96, args.steps)), flush=True)
+        return
+    if args.only == "diff":
+        print(json.dumps(bench_differentiable(args.diff_steps or 64, args.diff_batch)), flush=True)
+        return
 
     log(f"generating plume_{args.N}_mg")
     scene = scenes.plume_2048_mg(N=args.N)

@@ -85,6 +85,29 @@ __device__ __forceinline__ double sample_field(const T* __restrict__ f, int fh,
   return c0 + (c1 - c0) * ty;
 }
 
+// --------------------------------------------------------- XCD-aware order
+// Workgroups are handed to the chip's 8 XCDs round-robin by launch order, so
+// blocks id and id+1 run on different XCDs with separate L2s.  remap_block
+// turns the launch id into a tile id such that each XCD walks a contiguous
+// run of tiles (rows of tiles for a row-major grid): vertically adjacent
+// tiles, which share halo rows, then meet in the same L2.  Bijective on
+// [0, nb); the nb % 8 tail keeps its ids.
+constexpr int NUM_XCD = 8;
+
+__device__ __forceinline__ int remap_block(int id, int nb) {
+  const int full = nb - nb % NUM_XCD;
+  if (id >= full) return id;
+  return (id % NUM_XCD) * (full / NUM_XCD) + id / NUM_XCD;
+}
+
+// (bx, by) of the tile a 2-D launch's workgroup handles, in XCD order.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by) {
+  const int gx = (int)gridDim.x;
+  const int v = remap_block((int)(blockIdx.y * gridDim.x + blockIdx.x), gx * (int)gridDim.y);
+  by = v / gx;
+  bx = v - by * gx;
+}
+
 // --------------------------------------------------------- reductions
 // Wave64 sum with DPP-backed cross-lane moves (__shfl_xor lowers to
 // ds_swizzle / DPP on gfx950).  Fixed order => deterministic.

@@ -42,7 +42,9 @@ inline dim3 grid2d(int rows, int cols) { return dim3(cdiv(cols, BX), cdiv(rows, 
 // --------------------------------------------------------------- masks
 __global__ void k_face_masks(const uint8_t* __restrict__ solid, int H, int W,
                              uint8_t* __restrict__ fmu, uint8_t* __restrict__ fmv) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j < H && i <= W) {
     uint8_t m = 0;
     if (i >= 1 && i <= W - 1) m = !(solid[(size_t)j * W + i - 1] || solid[(size_t)j * W + i]);
@@ -57,7 +59,9 @@ __global__ void k_face_masks(const uint8_t* __restrict__ solid, int H, int W,
 
 __global__ void k_is_fluid(const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv,
                            int H, int W, uint8_t* __restrict__ out) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
   const size_t u0 = (size_t)j * (W + 1) + i;
   out[(size_t)j * W + i] = (fmu[u0] || fmu[u0 + 1] || fmv[(size_t)j * W + i] ||
@@ -66,7 +70,9 @@ __global__ void k_is_fluid(const uint8_t* __restrict__ fmu, const uint8_t* __res
 
 __global__ void k_cell_flags(const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv,
                              int H, int W, uint8_t* __restrict__ flags) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
   const size_t u0 = (size_t)j * (W + 1) + i;
   const uint8_t ml = fmu[u0], mr = fmu[u0 + 1];
@@ -95,7 +101,9 @@ __global__ void k_advect_scalar(const T* __restrict__ src, T* __restrict__ dst, 
                                 const T* __restrict__ v, int vh, int vw,
                                 const uint8_t* __restrict__ fluid, double dt, double inv,
                                 double scale, int apply_scale) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
   const size_t o = (size_t)j * W + i;
   if (!fluid[o]) { dst[o] = (T)0; return; }
@@ -143,7 +151,9 @@ template <typename T>
 __global__ void k_advect_u(const T* __restrict__ us, T* __restrict__ ud, int uh, int uw,
                            const T* __restrict__ v, int vh, int vw,
                            const uint8_t* __restrict__ fmu, double dt, double inv) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j < uh && i < uw) advect_u_at(us, ud, uh, uw, v, vh, vw, fmu, dt, inv, j, i);
 }
 
@@ -151,7 +161,9 @@ template <typename T>
 __global__ void k_advect_v(const T* __restrict__ vs, T* __restrict__ vd, int vh, int vw,
                            const T* __restrict__ u, int uh, int uw,
                            const uint8_t* __restrict__ fmv, double dt, double inv) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j < vh && i < vw) advect_v_at(vs, vd, vh, vw, u, uh, uw, fmv, dt, inv, j, i);
 }
 
@@ -162,7 +174,9 @@ __global__ void k_advect_velocity(const T* __restrict__ us, T* __restrict__ ud,
                                   const uint8_t* __restrict__ fmu,
                                   const uint8_t* __restrict__ fmv, int H, int W, double dt,
                                   double inv) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j < H && i <= W) advect_u_at(us, ud, H, W + 1, vs, H + 1, W, fmu, dt, inv, j, i);
   if (j <= H && i < W) advect_v_at(vs, vd, H + 1, W, us, H, W + 1, fmv, dt, inv, j, i);
 }
@@ -176,7 +190,9 @@ __global__ void k_apply_forces(T* __restrict__ v, int vh, int vw,
                                const T* __restrict__ rho, int H, int W,
                                const T* __restrict__ temp, int Ht, int Wt, double dt,
                                double g, double bc, double dc, double Ta) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= vh || i >= vw) return;
   float bf = 0.0f, tf = 0.0f;
   if (j < H && i < W) {
@@ -228,7 +244,9 @@ template <typename T>
 __global__ void k_divergence(const T* __restrict__ u, const T* __restrict__ v,
                              T* __restrict__ div, const uint8_t* __restrict__ fmu,
                              const uint8_t* __restrict__ fmv, int H, int W, double inv) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
   const size_t a = (size_t)j * (W + 1) + i;
   const size_t b = (size_t)j * W + i;
@@ -247,7 +265,9 @@ __global__ void k_pressure_gradient(T* __restrict__ u, T* __restrict__ v,
                                     const uint8_t* __restrict__ fmv,
                                     const uint8_t* __restrict__ solid, int H, int W,
                                     double scale) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j < H && i <= W) {
     const size_t o = (size_t)j * (W + 1) + i;
     T val;
@@ -289,7 +309,9 @@ __global__ void k_pressure_gradient(T* __restrict__ u, T* __restrict__ v,
 template <typename T>
 __global__ void k_enforce_boundaries(T* __restrict__ u, T* __restrict__ v,
                                      const uint8_t* __restrict__ solid, int H, int W) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j < H && i <= W) {
     const bool ls = i > 0 ? solid[(size_t)j * W + i - 1] == 1 : false;
     const bool rs = i < W ? solid[(size_t)j * W + i] == 1 : false;
@@ -309,7 +331,9 @@ __global__ void k_residual_masks(const T* __restrict__ p, const T* __restrict__ 
                                  const uint8_t* __restrict__ fmu,
                                  const uint8_t* __restrict__ fmv, int H, int W,
                                  double alpha, double beta, T* __restrict__ r) {
-  const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
   const size_t o = (size_t)j * W + i;
   if (!fluid[o]) { r[o] = (T)0; return; }

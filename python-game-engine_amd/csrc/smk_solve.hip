@@ -289,14 +289,16 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
 
 template <typename T, int MODE, int RH, int NT>
 __global__ __launch_bounds__(NT) void k_smooth_tile64(const SmoothArgs<T> a) {
-  smooth_tile64_body<T, MODE, RH, NT>(a, blockIdx.x, blockIdx.y,
-                                      blockIdx.y * gridDim.x + blockIdx.x);
+  int bx, by;
+  xcd_tile(bx, by);
+  smooth_tile64_body<T, MODE, RH, NT>(a, bx, by, by * gridDim.x + bx);
 }
 
 template <typename T, int RH, int NT>
 __global__ __launch_bounds__(NT) void k_smooth_fine(const SmoothArgs<T> a) {
-  smooth_tile64_body<T, MODE_RB, RH, NT>(a, blockIdx.x, blockIdx.y,
-                                         blockIdx.y * gridDim.x + blockIdx.x);
+  int bx, by;
+  xcd_tile(bx, by);
+  smooth_tile64_body<T, MODE_RB, RH, NT>(a, bx, by, by * gridDim.x + bx);
 }
 
 // Red-black smoothing with the region in registers.  A workgroup of NW
@@ -899,8 +901,10 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
                                                         const int* __restrict__ gate) {
   __shared__ double red[NB_T / 64];
   if (gate != nullptr && *gate == 0) return;
-  const int i = blockIdx.x * NB_X + (threadIdx.x & (NB_X - 1));
-  const int j = blockIdx.y * NB_Y + (threadIdx.x / NB_X);
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i = bx * NB_X + (threadIdx.x & (NB_X - 1));
+  const int j = by * NB_Y + (threadIdx.x / NB_X);
   double acc = 0.0;
   if (i < W && j < H) {
     const size_t o = (size_t)j * W + i;
@@ -918,7 +922,7 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
     if (defect != nullptr) defect[o] = (f & F_FLUID) ? (float)(rv / alpha) : 0.0f;
   }
   const double tot = block_sum<NB_T>(acc, red);
-  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = tot;
+  if (threadIdx.x == 0) partial[by * gridDim.x + bx] = tot;
 }
 
 // ||alpha*rhs||^2 partials over fluid cells (the p = 0 residual).
