@@ -12,8 +12,14 @@ A step is one full SmokeSimulation2D.substep (sources, buoyancy, advection of
 u, v and density, divergence, pressure solve, projection + walls, metrics
 divergence and residual) with all fields resident in HBM.  Two variants:
 ``fp64`` (FP64 V-cycles) and ``mixed`` (FP32 V-cycles inside FP64 defect
-correction, same FP64 tolerance).  ``value`` is the mixed variant's
-cell-steps/s; the fp64 figures are reported beside it.
+correction, same FP64 tolerance).  ``value`` is the faster variant's
+cell-steps/s (``config.variant`` names it); both are reported.
+
+The same line carries the two secondary workloads: ``batched_scenes`` (96
+uneven scenes stepped in one batch, FP32 V-cycles in FP64 refinement to
+1e-6) and ``differentiable`` (the 256^2 64-step rollout: forward vs forward +
+adjoint time, FP32 and FP64, and a batch of seeded targets); ``--only``
+runs one of them alone.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -318,15 +324,36 @@ def main():
         results[var] = r
         log(f"variant {var}: {json.dumps(r)}")
 
-    main_var = "mixed" if "mixed" in results else list(results)[0]
+    # the headline variant is the faster one; both are reported
+    main_var = min(results, key=lambda k: results[k]["ms_per_step"])
     m = results[main_var]
     copy_gbps = calibrate_copy()
-    dom = m["kernels"][0] if m["kernels"] else None
-    traffic = None
+    # the roofline kernel: the family carrying most of the step's algorithmic
+    # HBM bytes (for an HBM roofline the kernel that sets the step's traffic;
+    # its HIP-event time matches rocprof's kernel time within a few percent,
+    # while event pairs around short coarse-level launches add dispatch time)
+    dom = max(m["kernels"], key=lambda k: k["alg_bytes"]) if m["kernels"] else None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if dom is not None and os.path.exists(tfile):
+    tmap = {}
+    if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(dom["name"], {}).get("hbm_bytes_per_launch")
+            tmap = json.load(f)
+
+    def traffic_of(name):
+        return tmap.get(name, {}).get("hbm_bytes_per_launch")
+
+    def roofline_of(k):
+        return {"bound": "hbm", "kernel": k["name"], "achieved": k["alg_GBps"],
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": k["alg_GBps"] / PEAK_HBM_GBS,
+                "traffic": traffic_of(k["name"]), "avg_launch_us": k["avg_launch_us"],
+                "alg_bytes_per_launch": k["alg_bytes_per_launch"]}
+
+    traffic = traffic_of(dom["name"]) if dom else None
+    fine = {}
+    for var, r in results.items():
+        for k in r["kernels"]:
+            if k["name"].startswith("rb_smooth_fine"):
+                fine[var] = roofline_of(k)
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_steps} oracle steps")
@@ -361,13 +388,10 @@ def main():
                              "rediscretized coarse scale",
                    "variant": main_var},
         "variants": results,
-        "roofline": None if dom is None else {
-            "bound": "hbm", "kernel": dom["name"], "achieved": dom["alg_GBps"],
-            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": dom["alg_GBps"] / PEAK_HBM_GBS,
-            "traffic": traffic, "avg_launch_us": dom["avg_launch_us"],
-            "alg_bytes_per_launch": dom["alg_bytes_per_launch"],
-            "whole_step_alg_GBps": m["alg_GBps_whole_step"],
-            "copy_kernel_GBps": copy_gbps},
+        "roofline": None if dom is None else dict(
+            roofline_of(dom), whole_step_alg_GBps=m["alg_GBps_whole_step"],
+            copy_kernel_GBps=copy_gbps),
+        "fine_level_roofline": fine,
         "cpu_baseline": cpu,
     }
     if "fp64" in results and "mixed" in results:

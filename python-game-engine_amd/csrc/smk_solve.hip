@@ -1034,19 +1034,33 @@ struct ProfRec {
 };
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
+// Events are recycled through a pool: creating two per launch on the host
+// would slow submission enough to leave gaps inside the timed intervals of
+// short kernels.
+static std::vector<hipEvent_t> g_prof_pool;
+
+static hipEvent_t prof_event() {
+  if (!g_prof_pool.empty()) {
+    hipEvent_t e = g_prof_pool.back();
+    g_prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
 
 static hipEvent_t prof_begin(hipStream_t s) {
   if (!g_prof_on) return nullptr;
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  (void)hipEventRecord(e, s);
+  hipEvent_t e = prof_event();
+  if (e) (void)hipEventRecord(e, s);
   return e;
 }
 
 static void prof_end(hipEvent_t start, const char* name, double bytes, hipStream_t s) {
   if (start == nullptr) return;
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return;
+  hipEvent_t e = prof_event();
+  if (e == nullptr) return;
   (void)hipEventRecord(e, s);
   g_prof.push_back({name, start, e, bytes});
 }
@@ -1903,6 +1917,12 @@ extern "C" {
 
 int smk_profile_enable(int on) {
   g_prof_on = on != 0;
+  // warm the event pool so that profiled launches create none
+  while (g_prof_on && g_prof_pool.size() < 4096) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) break;
+    g_prof_pool.push_back(e);
+  }
   return SMK_OK;
 }
 
@@ -1927,8 +1947,8 @@ int smk_profile_read(smk_profile_record* out, int n, int reset) {
   }
   if (reset) {
     for (ProfRec& r : g_prof) {
-      (void)hipEventDestroy(r.start);
-      (void)hipEventDestroy(r.stop);
+      g_prof_pool.push_back(r.start);
+      g_prof_pool.push_back(r.stop);
     }
     g_prof.clear();
   }
