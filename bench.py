@@ -211,7 +211,7 @@ def bench_batched(count: int, steps: int) -> dict:
     return out
 
 
-def bench_differentiable(steps: int, batch: int) -> dict:
+def bench_differentiable(steps: int, batch: int, grad_check: bool = True) -> dict:
     """The 256^2 differentiable rollout: 64 steps, MG to 1e-8, controls u0,
     v0 (zero) and per-step strengths (1.0), loss 0.5||rho_T - target||^2
     with a Gaussian target; forward-only vs forward + adjoint (checkpoint
@@ -270,7 +270,59 @@ def bench_differentiable(steps: int, batch: int) -> dict:
             res["batch_targets"] = batch
             res["batch_ms"] = a.elapsed_time(c)
         out[name] = res
+    if grad_check:
+        out["gradient_check"] = _gradient_check(sc, steps)
     return out
+
+
+def _gradient_check(sc, steps: int, directions: int = 32, eps: float = 1e-6) -> dict:
+    """FP64 gradient at the rollout's starting controls vs central differences
+    along seeded random directions (solves run a fixed 30 V-cycles, converged
+    to round-off, so the loss is a smooth function of the controls), and the
+    FP32 gradient's relative error against the FP64 one."""
+    import numpy as np
+    import torch
+
+    from cfd_amd import diff
+
+    N = sc.H
+    grads = {}
+    for name, dt_, kw in (("fp64", torch.float64, {"tolerance": 0.0, "v_cycles": 30}),
+                          ("fp64_tol", torch.float64, {}), ("fp32", torch.float32, {})):
+        m = diff.from_scene(sc, dtype=dt_, **kw)
+        tgt = torch.as_tensor(diff.gaussian_target(N, N, sc.cell_size), dtype=dt_, device="cuda")
+        z = dict(dtype=dt_, device="cuda", requires_grad=True)
+        u0 = torch.zeros((N, N + 1), **z)
+        v0 = torch.zeros((N + 1, N), **z)
+        s = torch.ones(steps, **z)
+        rho, _, _ = m.rollout(u0, v0, s, steps, checkpoint_every=8)
+        diff.density_loss(rho, tgt).backward()
+        grads[name] = (u0.grad.double(), v0.grad.double(), s.grad.double())
+        if name == "fp64":
+            model, target = m, tgt
+    rng = np.random.default_rng(50)
+    errs = []
+    zero = dict(dtype=torch.float64, device="cuda")
+    for _ in range(directions):
+        du = torch.as_tensor(rng.standard_normal((N, N + 1)), **zero)
+        dv = torch.as_tensor(rng.standard_normal((N + 1, N)), **zero)
+        ds = torch.as_tensor(rng.standard_normal(steps), **zero)
+        gu, gv, gs = grads["fp64"]
+        adj = float((gu * du).sum() + (gv * dv).sum() + (gs * ds).sum())
+        fd = diff.directional_fd(model, torch.zeros((N, N + 1), **zero),
+                                 torch.zeros((N + 1, N), **zero), torch.ones(steps, **zero),
+                                 steps, target, du, dv, ds, eps)
+        errs.append(abs(fd - adj) / abs(fd))
+
+    def rel(a, b):
+        """max |a - b| / max |b| per control (u0, v0, strengths)"""
+        return {k: float((x - y).abs().max() / y.abs().max())
+                for k, x, y in zip(("u0", "v0", "strengths"), a, b)}
+
+    return {"directions": directions, "eps": eps,
+            "fd_max_rel_err": max(errs), "fd_median_rel_err": float(np.median(errs)),
+            "tolerance_1e-8_vs_converged_rel_err": rel(grads["fp64_tol"], grads["fp64"]),
+            "fp32_vs_fp64_rel_err": rel(grads["fp32"], grads["fp64"])}
 
 
 def main():
