@@ -160,6 +160,50 @@ def cpu_baseline(scene, sample_steps: int):
                        f"{cyc / sample_steps:.1f} V-cycles/step, {dt:.1f} s")}
 
 
+def _time_steps(sim, scene, steps: int, warmup: int) -> tuple:
+    import torch
+
+    for _ in range(warmup):
+        sim.substep(scene.dt)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    its = []
+    e0.record()
+    for _ in range(steps):
+        sim.substep(scene.dt)
+        its.append(sim.last_solve_info.get("iterations", 0))
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, sum(its) / max(len(its), 1)
+
+
+def bench_small_configs(steps_c1: int = 100, steps_c2: int = 20) -> dict:
+    """The 128^2 Jacobi-200 plume (100 steps) and the 512^2 RB-SOR plume
+    (omega 1.9 to 1e-6, FP64 and FP32 sweeps in FP64 defect correction)."""
+    from cfd_amd import scenes
+
+    out = {}
+    sc = scenes.plume_128_jacobi()
+    sim = scenes.make_simulation(sc, dtype="float64", device="cuda")
+    ms, its = _time_steps(sim, sc, steps_c1, 2)
+    out["plume_128_jacobi"] = {"steps": steps_c1, "ms_per_step": ms,
+                               "cell_steps_per_s": sc.cells / (ms / 1e3),
+                               "jacobi_iterations_per_step": its}
+    sc = scenes.plume_512_rbsor()
+    res = {}
+    for prec in ("native", "mixed"):
+        sim = scenes.make_simulation(sc, dtype="float64", device="cuda",
+                                     solver_overrides={"precision": prec})
+        ms, its = _time_steps(sim, sc, steps_c2, 2)
+        res["fp64" if prec == "native" else "mixed"] = {
+            "ms_per_step": ms, "cell_steps_per_s": sc.cells / (ms / 1e3),
+            "sweeps_per_step": its, "rel_residual_last": sim.last_solve_info.get("rel_residual")}
+    res["steps"] = steps_c2
+    res["mixed_speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["mixed"]["ms_per_step"]
+    out["plume_512_rbsor"] = res
+    return out
+
+
 def bench_batched(count: int, steps: int) -> dict:
     """The uneven-scenes workload: ``count`` scenes (N log-uniform in
     [64, 1024], multiple of 16, aspect 1:1 or 2:1, seeds 4000+s), one
@@ -339,7 +383,9 @@ def main():
     ap.add_argument("--diff-steps", type=int, default=64,
                     help="steps of the differentiable rollout (0 skips it)")
     ap.add_argument("--diff-batch", type=int, default=16)
-    ap.add_argument("--only", choices=("batched", "diff"), default=None,
+    ap.add_argument("--no-small-configs", action="store_true",
+                    help="skip the 128^2 Jacobi and 512^2 RB-SOR lines")
+    ap.add_argument("--only", choices=("batched", "diff", "small"), default=None,
                     help="run one secondary workload alone and print its JSON (profiling runs)")
     args = ap.parse_args()
 
@@ -358,6 +404,9 @@ def main():
 
     if args.only == "batched":
         print(json.dumps(bench_batched(args.batched_scenes or 96, args.steps)), flush=True)
+        return
+    if args.only == "small":
+        print(json.dumps(bench_small_configs()), flush=True)
         return
     if args.only == "diff":
         print(json.dumps(bench_differentiable(args.diff_steps or 64, args.diff_batch)), flush=True)
@@ -413,6 +462,10 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     extra = {}
+    if not args.no_small_configs:
+        log("128^2 Jacobi and 512^2 RB-SOR plumes")
+        extra.update(bench_small_configs())
+        log(f"small configs: {json.dumps({k: extra[k] for k in ('plume_128_jacobi', 'plume_512_rbsor')})}")
     if args.batched_scenes > 0:
         log(f"batched scenes: {args.batched_scenes}")
         extra["batched_scenes"] = bench_batched(args.batched_scenes, 20)

@@ -172,6 +172,17 @@ int smk_rbsor_solve(void* p, const void* div, const uint8_t* fmu,
                     double tol, int check_every, void* ws, size_t ws_bytes,
                     smk_solve_info* info, int dtype, void* stream);
 
+/* The same solver with FP32 sweeps inside FP64 defect correction (no
+ * reference counterpart): p and div are float64; every check_every sweeps
+ * the FP64 residual (and its norm) is formed, the sweeps run in FP32 from
+ * zero on the residual equation, and p += e in FP64.  Same tolerance rule
+ * and sweep accounting as smk_rbsor_solve; workspace from
+ * smk_solver_workspace_bytes(H, W, SMK_F64). */
+int smk_rbsor_solve_mixed(void* p, const void* div, const uint8_t* fmu,
+                          const uint8_t* fmv, int H, int W, double rho,
+                          double h, double dt, int iterations, double omega,
+                          double tol, int check_every, void* ws,
+                          size_t ws_bytes, smk_solve_info* info, void* stream);
 /* ||r||^2 over fluid cells of the residual of residual.py:6-42 with
  * reciprocal_beta = 1/4, reduced on the device in a fixed order and read
  * back into *norm2 (synchronises the stream). */

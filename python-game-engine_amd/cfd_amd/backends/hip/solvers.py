@@ -64,21 +64,35 @@ def jacobi_pressure_solver(pressure, divergence, face_mask_u, face_mask_v, fluid
 def red_black_sor_gauss_seidel_pressure_solver(pressure, divergence, face_mask_u, face_mask_v,
                                                fluid_density, cell_size, dt, iterations,
                                                omega=0.0, tolerance=0.0, check_every=20,
-                                               **_ignored):
+                                               precision="native", **_ignored):
     """red_black_sor_gauss_seidel.py:30-42.
 
     omega <= 0 or > 2 selects calculateOptimalOmega(max(H, W)).  With
     tolerance > 0 the sweeps stop once ||r||/||alpha*div|| <= tolerance
-    (checked every ``check_every`` sweeps; ``iterations`` is then the cap)."""
+    (checked every ``check_every`` sweeps; ``iterations`` is then the cap).
+    ``precision="mixed"`` (float64 fields only) runs the sweeps in FP32 inside
+    FP64 defect correction, one correction per ``check_every`` sweeps."""
     H, W = _check(pressure, divergence, face_mask_u, face_mask_v)
     code = dtype_code(pressure)
+    if precision not in _PREC:
+        raise ValueError(f"precision must be one of {sorted(_PREC)}, got {precision!r}")
     nb = lib().smk_solver_workspace_bytes(H, W, code)
     ws = _workspace(pressure.device, nb)
     info = SolveInfo()
-    call("smk_rbsor_solve", pressure.data_ptr(), divergence.data_ptr(), face_mask_u.data_ptr(),
-         face_mask_v.data_ptr(), H, W, float(fluid_density), float(cell_size), float(dt),
-         int(iterations), float(omega), float(tolerance), int(check_every), ws.data_ptr(),
-         ws.numel(), ctypes.byref(info), code, stream_of(pressure))
+    if precision == "mixed":
+        if pressure.dtype != torch.float64:
+            raise ValueError("precision='mixed' needs float64 pressure and divergence")
+        call("smk_rbsor_solve_mixed", pressure.data_ptr(), divergence.data_ptr(),
+             face_mask_u.data_ptr(), face_mask_v.data_ptr(), H, W, float(fluid_density),
+             float(cell_size), float(dt), int(iterations), float(omega), float(tolerance),
+             int(check_every), ws.data_ptr(), ws.numel(), ctypes.byref(info),
+             stream_of(pressure))
+    else:
+        call("smk_rbsor_solve", pressure.data_ptr(), divergence.data_ptr(),
+             face_mask_u.data_ptr(), face_mask_v.data_ptr(), H, W, float(fluid_density),
+             float(cell_size), float(dt), int(iterations), float(omega), float(tolerance),
+             int(check_every), ws.data_ptr(), ws.numel(), ctypes.byref(info), code,
+             stream_of(pressure))
     LAST_SOLVE_INFO.clear()
     LAST_SOLVE_INFO.update(method="rb_sor_gauss_seidel", **info.as_dict())
 

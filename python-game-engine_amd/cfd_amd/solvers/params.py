@@ -46,6 +46,7 @@ class SORParams:
     omega: float = 0.0  # 0.0 -> automatic optimal omega
     tolerance: float = 0.0  # new
     check_every: int = 20  # new
+    precision: str = "native"  # new: "mixed" = FP32 sweeps in FP64 defect correction
 
     def string(self):
         return f"SORParams_iters_{self.iterations}_omega_{self.omega}"

@@ -1235,13 +1235,18 @@ struct SolverWs {
   void* ptmp;
   double* partial;
   double* result;  // [0] = ||r||^2, [1] = ||alpha b||^2
+  SolveState* state;
+  float* e32[2];   // FP32 correction ping-pong (mixed precision, FP64 only)
+  float* defect32;
   size_t bytes;
 };
 
 inline size_t solver_ws_bytes(int H, int W, int dtype) {
   const size_t n = (size_t)H * W;
   const size_t es = dtype == SMK_F64 ? 8 : 4;
-  return align256(n) + align256(n * es) + align256((size_t)norm_blocks(H, W) * 8) + 256;
+  const size_t mixed = dtype == SMK_F64 ? 3 * align256(n * 4) : 0;
+  return align256(n) + align256(n * es) + align256((size_t)norm_blocks(H, W) * 8) + 256 + 256 +
+         mixed;
 }
 
 inline SolverWs carve_ws(void* ws, int H, int W, int dtype) {
@@ -1256,6 +1261,17 @@ inline SolverWs carve_ws(void* ws, int H, int W, int dtype) {
   w.partial = reinterpret_cast<double*>(b);
   b += align256((size_t)norm_blocks(H, W) * 8);
   w.result = reinterpret_cast<double*>(b);
+  b += 256;
+  w.state = reinterpret_cast<SolveState*>(b);
+  b += 256;
+  w.e32[0] = w.e32[1] = w.defect32 = nullptr;
+  if (dtype == SMK_F64) {
+    w.e32[0] = reinterpret_cast<float*>(b);
+    b += align256(n * 4);
+    w.e32[1] = reinterpret_cast<float*>(b);
+    b += align256(n * 4);
+    w.defect32 = reinterpret_cast<float*>(b);
+  }
   w.bytes = solver_ws_bytes(H, W, dtype);
   return w;
 }
@@ -1267,7 +1283,8 @@ inline SolverWs carve_ws(void* ws, int H, int W, int dtype) {
 template <typename T>
 int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
                const uint8_t* flags, int H, int W, double alpha, double omega,
-               int npass_total, int first_colour, hipStream_t s) {
+               int npass_total, int first_colour, hipStream_t s, const int* gate = nullptr,
+               bool zero_first = false) {
   if (npass_total <= 0) {
     if (src != dst_final)
       SMK_CHECK_HIP(hipMemcpyAsync(dst_final, src, (size_t)H * W * sizeof(T),
@@ -1325,8 +1342,9 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
     a.npass = chunks[q];
     a.first_colour = colour;
     a.R = chunks[q];
-    a.zero_init = 0;
+    a.zero_init = (zero_first && q == 0) ? 1 : 0;
     a.coarse_div = 1.0;
+    a.gate = gate;
     int rc = launch_tiled<T>(mode, a, s, true);
     if (rc) return rc;
     colour = (colour + chunks[q]) & 1;
@@ -1385,10 +1403,89 @@ inline double optimal_omega(int size) {
   return w < 1.95 ? w : 1.95;
 }
 
+// Queue gated iterations -- iter(k) enqueues iteration k, ending with a
+// k_check on d_state -- until the device reports the solve done.  The
+// outcome of iteration k-1 is read (from a pinned ring slot) while iteration
+// k is already queued, so the GPU never waits for the host; the one
+// iteration queued after the last useful one is gated off and does nothing.
+template <typename F>
+int run_gated_ring(SolveState* d_state, SolveState* h_ring, hipEvent_t* ev, int max_iter,
+                   F&& iter, SolveState* fin, hipStream_t s) {
+  for (int k = 0; k <= max_iter; ++k) {
+    int rc = iter(k);
+    if (rc) return rc;
+    const int slot = k & 3;
+    SMK_CHECK_HIP(hipMemcpyAsync(&h_ring[slot], d_state, sizeof(SolveState),
+                                 hipMemcpyDeviceToHost, s));
+    SMK_CHECK_HIP(hipEventRecord(ev[slot], s));
+    if (k >= 1) {
+      const int prev = (k - 1) & 3;
+      SMK_CHECK_HIP(hipEventSynchronize(ev[prev]));
+      if (h_ring[prev].active == 0) break;
+    }
+  }
+  SMK_CHECK_HIP(hipStreamSynchronize(s));
+  SMK_CHECK_HIP(hipMemcpy(fin, d_state, sizeof(SolveState), hipMemcpyDeviceToHost));
+  return SMK_OK;
+}
+
+// Pinned read-back ring and events of the single-grid solvers (which take a
+// caller workspace, not a handle): one set per host thread, kept until exit.
+struct GateRing {
+  SolveState* h_ring = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+static int thread_gate_ring(GateRing** out) {
+  thread_local GateRing g;
+  if (g.h_ring == nullptr) {
+    SMK_CHECK_HIP(hipHostMalloc((void**)&g.h_ring, 4 * sizeof(SolveState), hipHostMallocDefault));
+    for (hipEvent_t& e : g.ev) SMK_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  *out = &g;
+  return SMK_OK;
+}
+
+// Tolerance-driven sweeps, gated: iteration 0 checks the initial guess,
+// iteration k >= 1 runs chunk k of `check_every` sweeps (the last chunk may
+// be shorter) then checks.  k_check's cap is the number of chunks.
+template <typename F>
+int sweeps_gated(SolverWs& w, int iterations, int check_every, double tol, F&& chunk,
+                 const double* b2, int nparts, int* done_out, double* rel_out, hipStream_t s) {
+  GateRing* g = nullptr;
+  int rc = thread_gate_ring(&g);
+  if (rc) return rc;
+  const int nchunks = (iterations + check_every - 1) / check_every;
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, s, w.state);
+  rc = check_launch("k_state_init");
+  if (rc) return rc;
+  SolveState fin{};
+  rc = run_gated_ring(w.state, g->h_ring, g->ev, nchunks + 1, [&](int k) -> int {
+    int r = SMK_OK;
+    if (k > 0) {
+      const int start = (k - 1) * check_every;
+      const int n = iterations - start < check_every ? iterations - start : check_every;
+      if (n > 0) r = chunk(n, &w.state->active);
+    }
+    if (r) return r;
+    r = chunk(0, &w.state->active);  // the residual partials
+    if (r) return r;
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, w.partial, nparts, b2, w.state, tol,
+                       nchunks, 1);
+    return check_launch("k_check");
+  }, &fin, s);
+  if (rc) return rc;
+  const long done = (long)fin.cycles * check_every;
+  *done_out = (int)(done < iterations ? done : iterations);
+  *rel_out = fin.rel;
+  return SMK_OK;
+}
+
 template <typename T>
 int rbsor_solve_t(T* p, const T* div, const uint8_t* fmu, const uint8_t* fmv, int H, int W,
                   double rho, double h, double dt, int iterations, double omega, double tol,
-                  int check_every, void* ws, smk_solve_info* info, hipStream_t s) {
+                  int check_every, void* ws, smk_solve_info* info, hipStream_t s,
+                  bool gated = true) {
   SolverWs w = carve_ws(ws, H, W, sizeof(T) == 8 ? SMK_F64 : SMK_F32);
   int rc = smk_cell_flags(fmu, fmv, H, W, w.flags, s);
   if (rc) return rc;
@@ -1409,6 +1506,23 @@ int rbsor_solve_t(T* p, const T* div, const uint8_t* fmu, const uint8_t* fmv, in
   if (check_every < 1) check_every = 1;
   rc = rhs_norm2_dev<T>(div, w.flags, H, W, alpha, w.partial, w.result, 1, s);
   if (rc) return rc;
+  if (gated) {
+    int done = 0;
+    double rel = -1.0;
+    rc = sweeps_gated(w, iterations, check_every, tol, [&](int n, const int* gate) -> int {
+      if (n == 0)
+        return residual_partials_dev<T>(p, div, w.flags, H, W, alpha, w.partial, nullptr, gate, s);
+      return run_passes<T>(MODE_RB, p, tmp, p, div, w.flags, H, W, alpha, omega, 2 * n, 0, s,
+                           gate);
+    }, w.result + 1, norm_blocks(H, W), &done, &rel, s);
+    if (rc) return rc;
+    if (info) {
+      info->iterations = done;
+      info->converged = rel <= tol;
+      info->rel_residual = rel;
+    }
+    return SMK_OK;
+  }
   rc = residual_norm2_dev<T>(p, div, w.flags, H, W, alpha, w.partial, w.result, 0, nullptr, s);
   if (rc) return rc;
   double host[2];
@@ -1434,6 +1548,51 @@ int rbsor_solve_t(T* p, const T* div, const uint8_t* fmu, const uint8_t* fmv, in
     info->iterations = done;
     info->converged = rel <= tol;
     info->rel_residual = rel;
+  }
+  return SMK_OK;
+}
+
+// FP32 red-black SOR sweeps as the inner solver of FP64 defect correction:
+// per chunk of `check_every` sweeps, r = alpha*div - A p in FP64 (with its
+// norm), defect = r/alpha in FP32, e = the chunk's sweeps from zero on
+// A e = alpha*defect in FP32, p += e in FP64.  A stationary iteration from
+// zero on the residual equation is the same linear map as continuing it from
+// p, so the sweep count to a tolerance matches the FP64 solver's.
+int rbsor_solve_mixed(double* p, const double* div, const uint8_t* fmu, const uint8_t* fmv,
+                      int H, int W, double rho, double h, double dt, int iterations,
+                      double omega, double tol, int check_every, void* ws, smk_solve_info* info,
+                      hipStream_t s) {
+  SolverWs w = carve_ws(ws, H, W, SMK_F64);
+  int rc = smk_cell_flags(fmu, fmv, H, W, w.flags, s);
+  if (rc) return rc;
+  const double alpha = calc_alpha(h, rho, dt);
+  if (omega <= 0.0 || omega > 2.0) omega = optimal_omega(H > W ? H : W);
+  if (check_every < 1) check_every = 1;
+  const long n = (long)H * W;
+  rc = rhs_norm2_dev<double>(div, w.flags, H, W, alpha, w.partial, w.result, 1, s);
+  if (rc) return rc;
+  int done = 0;
+  double rel = -1.0;
+  rc = sweeps_gated(w, iterations, check_every, tol, [&](int k, const int* gate) -> int {
+    if (k == 0)
+      return residual_partials_dev<double>(p, div, w.flags, H, W, alpha, w.partial, w.defect32,
+                                           gate, s);
+    int r = run_passes<float>(MODE_RB, w.e32[0], w.e32[1], w.e32[0], w.defect32, w.flags, H, W,
+                              alpha, omega, 2 * k, 0, s, gate, true);
+    if (r) return r;
+    hipEvent_t ev = prof_begin(s);
+    hipLaunchKernelGGL(k_add_correction<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p,
+                       w.e32[0], n, gate);
+    prof_end(ev, "correct_fp64", 20.0 * (double)n, s);
+    return check_launch("k_add_correction");
+  }, w.result + 1, norm_blocks(H, W), &done, &rel, s);
+  if (rc) return rc;
+  if (info) {
+    info->iterations = done;
+    info->converged = tol > 0.0 && rel <= tol;
+    info->rel_residual = rel;
+    info->inner_cycles = 0;
+    info->refinements = (done + check_every - 1) / check_every;
   }
   return SMK_OK;
 }
@@ -1692,22 +1851,7 @@ int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, do
 // iteration queued after the last useful one is gated off and does nothing.
 template <typename F>
 int run_gated(smk_mg* mg, int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
-  for (int k = 0; k <= max_iter; ++k) {
-    int rc = iter(k);
-    if (rc) return rc;
-    const int slot = k & 3;
-    SMK_CHECK_HIP(hipMemcpyAsync(&mg->h_ring[slot], mg->d_state, sizeof(SolveState),
-                                 hipMemcpyDeviceToHost, s));
-    SMK_CHECK_HIP(hipEventRecord(mg->ev[slot], s));
-    if (k >= 1) {
-      const int prev = (k - 1) & 3;
-      SMK_CHECK_HIP(hipEventSynchronize(mg->ev[prev]));
-      if (mg->h_ring[prev].active == 0) break;
-    }
-  }
-  SMK_CHECK_HIP(hipStreamSynchronize(s));
-  SMK_CHECK_HIP(hipMemcpy(fin, mg->d_state, sizeof(SolveState), hipMemcpyDeviceToHost));
-  return SMK_OK;
+  return run_gated_ring(mg->d_state, mg->h_ring, mg->ev, max_iter, iter, fin, s);
 }
 
 // Gated iterations need the finest level's ping-pong flips per V-cycle to be
@@ -2023,6 +2167,19 @@ int smk_rbsor_solve(void* p, const void* div, const uint8_t* fmu, const uint8_t*
                                  iterations, omega, tol, check_every, ws, info, s);
   return rbsor_solve_t<float>((float*)p, (const float*)div, fmu, fmv, H, W, rho, h, dt,
                               iterations, omega, tol, check_every, ws, info, s);
+}
+
+int smk_rbsor_solve_mixed(void* p, const void* div, const uint8_t* fmu, const uint8_t* fmv,
+                          int H, int W, double rho, double h, double dt, int iterations,
+                          double omega, double tol, int check_every, void* ws, size_t ws_bytes,
+                          smk_solve_info* info, void* stream) {
+  SMK_REQUIRE(p && div && fmu && fmv && ws, "smk_rbsor_solve_mixed: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0 && iterations >= 0, "smk_rbsor_solve_mixed: bad arguments");
+  SMK_REQUIRE(ws_bytes >= solver_ws_bytes(H, W, SMK_F64),
+              "smk_rbsor_solve_mixed: workspace too small (%zu < %zu)", ws_bytes,
+              solver_ws_bytes(H, W, SMK_F64));
+  return rbsor_solve_mixed((double*)p, (const double*)div, fmu, fmv, H, W, rho, h, dt,
+                           iterations, omega, tol, check_every, ws, info, as_stream(stream));
 }
 
 int smk_residual_norm2(const void* p, const void* div, const uint8_t* fmu, const uint8_t* fmv,

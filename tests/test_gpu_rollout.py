@@ -98,3 +98,40 @@ def test_full_size_multigrid_step_properties():
         assert after < 1e-4 * before
         outs.append(sim.pressure.cpu().numpy())
     np.testing.assert_array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("N", [96, 200])
+def test_mixed_rbsor_matches_fp64_to_tolerance(N):
+    """FP32 red-black SOR sweeps inside FP64 defect correction reach the same
+    tolerance in about the same number of sweeps as the FP64 solver and give
+    the same pressure to within the tolerance."""
+    import torch
+
+    from cfd_amd.backends.hip import solvers as S
+
+    rng = np.random.default_rng(N)
+    solid = np.zeros((N, N), dtype=bool)
+    solid[0, :] = solid[-1, :] = solid[:, 0] = solid[:, -1] = True
+    fmu = np.zeros((N, N + 1), dtype=bool)
+    fmv = np.zeros((N + 1, N), dtype=bool)
+    fmu[:, 1:-1] = ~(solid[:, :-1] | solid[:, 1:])
+    fmv[1:-1, :] = ~(solid[:-1, :] | solid[1:, :])
+    # consistent Neumann problem: zero on the solid ring, zero mean on fluid cells
+    div = rng.standard_normal((N, N))
+    div[solid] = 0.0
+    div[~solid] -= div[~solid].mean()
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    out = {}
+    for prec in ("native", "mixed"):
+        p = torch.zeros((N, N), dtype=torch.float64, device="cuda")
+        S.red_black_sor_gauss_seidel_pressure_solver(
+            p, dev(div), dev(fmu), dev(fmv), 1.0, 1.0 / N, 0.005, iterations=20000, omega=1.9,
+            tolerance=1e-7, check_every=20, precision=prec)
+        torch.cuda.synchronize()
+        out[prec] = (p.cpu().numpy(), dict(S.LAST_SOLVE_INFO))
+    (p64, i64), (p32, i32) = out["native"], out["mixed"]
+    assert i64["converged"] and i32["converged"]
+    assert i32["rel_residual"] <= 1e-7
+    assert abs(i32["iterations"] - i64["iterations"]) <= 0.1 * i64["iterations"] + 40
+    a, b = p64 - p64.mean(), p32 - p32.mean()
+    assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()
