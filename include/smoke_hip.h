@@ -369,6 +369,17 @@ int smk_batch_set_tuning(smk_batch* b, int tile_rows, int smoother_iterations,
                          int persistent_blocks, void* stream);
 int smk_batch_destroy(smk_batch* b);
 
+/* ------------------------------------------------- profiling metrics */
+/* profiling/config.py:268-301 reducers over profiling/metrics.py:7-28
+ * scopes, on the device: out18[scope * 6 + reducer] (float64, device
+ * memory), scopes all / interior ([1:-1, 1:-1]) / boundary, reducers rms /
+ * mean / mean_abs / max / max_abs / std (numpy semantics, ddof 0; NaN for
+ * an empty scope).  field is rows x cols. */
+size_t smk_field_stats_workspace_bytes(void);
+int smk_field_stats(const void* field, int rows, int cols, void* out18,
+                    void* workspace, size_t workspace_bytes, int dtype,
+                    void* stream);
+
 /* ------------------------------------------------- profiling / tuning */
 /* No reference counterpart: measurement and tuning hooks.
  * smk_profile_enable(1) makes every solver launch record a HIP event pair on

@@ -141,6 +141,8 @@ SIGNATURES = {
     "smk_set_smoother_variant": (_i, [_i, _i, _i]),
     "smk_get_smoother_variant": (_i, [_i, _i]),
     "smk_adjoint_workspace_bytes": (_sz, [_i, _i, _i]),
+    "smk_field_stats_workspace_bytes": (_sz, []),
+    "smk_field_stats": (_i, [_vp, _i, _i, _vp, _vp, _sz, _i, _vp]),
     "smk_batch_create": (_i, [ctypes.POINTER(BatchScene), _i, ctypes.POINTER(BatchParams), _vp,
                               ctypes.POINTER(_vp)]),
     "smk_batch_step": (_i, [_vp, ctypes.POINTER(BatchInfo), _vp]),
