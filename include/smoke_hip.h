@@ -172,6 +172,14 @@ int smk_rbsor_solve(void* p, const void* div, const uint8_t* fmu,
                     double tol, int check_every, void* ws, size_t ws_bytes,
                     smk_solve_info* info, int dtype, void* stream);
 
+/* gauss_seidel.py:8-48 gauss_seidel_pressure_solver (solver key
+ * "gauss_seidel"): `iterations` lexicographic sweeps (row by row, left to
+ * right, in place), run as a wavefront on one workgroup; identical to the
+ * sequential sweep.  Workspace as for the other single-grid solvers. */
+int smk_gs_solve(void* p, const void* div, const uint8_t* fmu,
+                 const uint8_t* fmv, int H, int W, double rho, double h,
+                 double dt, int iterations, void* ws, size_t ws_bytes,
+                 int dtype, void* stream);
 /* The same solver with FP32 sweeps inside FP64 defect correction (no
  * reference counterpart): p and div are float64; every check_every sweeps
  * the FP64 residual (and its norm) is formed, the sweeps run in FP32 from

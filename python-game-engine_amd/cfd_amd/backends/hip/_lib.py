@@ -127,6 +127,7 @@ SIGNATURES = {
     "smk_jacobi_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _vp, _sz, _i, _vp]),
     "smk_rbsor_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _d, _d, _i, _vp, _sz,
                              ctypes.POINTER(SolveInfo), _i, _vp]),
+    "smk_gs_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _vp, _sz, _i, _vp]),
     "smk_rbsor_solve_mixed": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _d, _d, _i, _vp,
                                    _sz, ctypes.POINTER(SolveInfo), _vp]),
     "smk_residual_norm2": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _sz,

@@ -97,6 +97,21 @@ def red_black_sor_gauss_seidel_pressure_solver(pressure, divergence, face_mask_u
     LAST_SOLVE_INFO.update(method="rb_sor_gauss_seidel", **info.as_dict())
 
 
+def gauss_seidel_pressure_solver(pressure, divergence, face_mask_u, face_mask_v, fluid_density,
+                                 cell_size, dt, iterations, **_ignored):
+    """gauss_seidel.py:29-48: ``iterations`` lexicographic sweeps, run as a
+    wavefront (every cell of one time step i + j + 2*sweep at once), which
+    reproduces the serial sweep exactly."""
+    H, W = _check(pressure, divergence, face_mask_u, face_mask_v)
+    code = dtype_code(pressure)
+    ws = _workspace(pressure.device, lib().smk_solver_workspace_bytes(H, W, code))
+    call("smk_gs_solve", pressure.data_ptr(), divergence.data_ptr(), face_mask_u.data_ptr(),
+         face_mask_v.data_ptr(), H, W, float(fluid_density), float(cell_size), float(dt),
+         int(iterations), ws.data_ptr(), ws.numel(), code, stream_of(pressure))
+    LAST_SOLVE_INFO.clear()
+    LAST_SOLVE_INFO.update(method="gauss_seidel", iterations=int(iterations))
+
+
 def residual_norm2(pressure, divergence, face_mask_u, face_mask_v, alpha) -> float:
     """||r||^2 over fluid cells (r of residual.py:6-42 with 1/beta = 1/4),
     reduced on the device; synchronises."""

@@ -1552,6 +1552,65 @@ int rbsor_solve_t(T* p, const T* div, const uint8_t* fmu, const uint8_t* fmv, in
   return SMK_OK;
 }
 
+// gauss_seidel.py:8-27 -- lexicographic Gauss-Seidel, as a wavefront.
+// In sweep s, cell (j, i) reads the new values of (j, i-1) and (j-1, i) and
+// the previous sweep's (j, i+1) and (j+1, i).  With t(s, j, i) = i + j + 2s
+// every one of those is produced at time t-1, and no two cells of one time
+// step are neighbours, so all cells of a time step update in place at once
+// and each reads exactly the operands the serial loop gives it: the result
+// is the reference's sequential sweep, bit for bit.  One workgroup walks the
+// (H-1)+(W-1)+2(S-1)+1 time steps with a barrier between them.
+constexpr int GS_NT = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(GS_NT) void k_gs_wavefront(T* __restrict__ p,
+                                                        const T* __restrict__ div,
+                                                        const uint8_t* __restrict__ flags, int H,
+                                                        int W, double alpha, int sweeps) {
+  const int tmax = (H - 1) + (W - 1) + 2 * (sweeps - 1);
+  for (int t = 0; t <= tmax; ++t) {
+    const int over = t - (H - 1) - (W - 1);
+    const int s_lo = over <= 0 ? 0 : (over + 1) / 2;
+    const int s_hi = (t / 2) < (sweeps - 1) ? (t / 2) : (sweeps - 1);
+    for (int sw = s_lo; sw <= s_hi; ++sw) {
+      const int d = t - 2 * sw;
+      const int j_lo = d - (W - 1) > 0 ? d - (W - 1) : 0;
+      const int j_hi = d < H - 1 ? d : H - 1;
+      for (int j = j_lo + (int)threadIdx.x; j <= j_hi; j += GS_NT) {
+        const int i = d - j;
+        const size_t o = (size_t)j * W + i;
+        const uint8_t f = flags[o];
+        if (!(f & F_FLUID)) continue;
+        const T ce = p[o];
+        const T l = (f & F_LEFT) ? p[o - 1] : ce;
+        const T r = (f & F_RIGHT) ? p[o + 1] : ce;
+        const T dn = (f & F_DOWN) ? p[o - W] : ce;
+        const T up = (f & F_UP) ? p[o + W] : ce;
+        p[o] = (T)poisson_new(l, r, dn, up, div[o], alpha);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int gs_solve(void* p, const void* div, const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+             double rho, double h, double dt, int iterations, void* ws, int dtype,
+             hipStream_t s) {
+  SolverWs w = carve_ws(ws, H, W, dtype);
+  int rc = smk_cell_flags(fmu, fmv, H, W, w.flags, s);
+  if (rc || iterations <= 0) return rc;
+  const double alpha = calc_alpha(h, rho, dt);
+  hipEvent_t ev = prof_begin(s);
+  if (dtype == SMK_F64)
+    hipLaunchKernelGGL(k_gs_wavefront<double>, dim3(1), dim3(GS_NT), 0, s, (double*)p,
+                       (const double*)div, w.flags, H, W, alpha, iterations);
+  else
+    hipLaunchKernelGGL(k_gs_wavefront<float>, dim3(1), dim3(GS_NT), 0, s, (float*)p,
+                       (const float*)div, w.flags, H, W, alpha, iterations);
+  prof_end(ev, "gauss_seidel", 3.0 * (dtype == SMK_F64 ? 8.0 : 4.0) * H * W * iterations, s);
+  return check_launch("k_gs_wavefront");
+}
+
 // FP32 red-black SOR sweeps as the inner solver of FP64 defect correction:
 // per chunk of `check_every` sweeps, r = alpha*div - A p in FP64 (with its
 // norm), defect = r/alpha in FP32, e = the chunk's sweeps from zero on
@@ -2167,6 +2226,18 @@ int smk_rbsor_solve(void* p, const void* div, const uint8_t* fmu, const uint8_t*
                                  iterations, omega, tol, check_every, ws, info, s);
   return rbsor_solve_t<float>((float*)p, (const float*)div, fmu, fmv, H, W, rho, h, dt,
                               iterations, omega, tol, check_every, ws, info, s);
+}
+
+int smk_gs_solve(void* p, const void* div, const uint8_t* fmu, const uint8_t* fmv, int H, int W,
+                 double rho, double h, double dt, int iterations, void* ws, size_t ws_bytes,
+                 int dtype, void* stream) {
+  SMK_REQUIRE(p && div && fmu && fmv && ws, "smk_gs_solve: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0 && iterations >= 0, "smk_gs_solve: bad arguments");
+  SMK_REQUIRE(dtype == SMK_F64 || dtype == SMK_F32, "smk_gs_solve: bad dtype %d", dtype);
+  SMK_REQUIRE(ws_bytes >= solver_ws_bytes(H, W, dtype),
+              "smk_gs_solve: workspace too small (%zu < %zu)", ws_bytes,
+              solver_ws_bytes(H, W, dtype));
+  return gs_solve(p, div, fmu, fmv, H, W, rho, h, dt, iterations, ws, dtype, as_stream(stream));
 }
 
 int smk_rbsor_solve_mixed(void* p, const void* div, const uint8_t* fmu, const uint8_t* fmv,

@@ -16,7 +16,7 @@ OPS = ("advect_scalar_field", "advect_velocity_u", "advect_velocity_v", "apply_f
        "cool_temperature", "compute_divergence", "apply_pressure_gradient",
        "enforce_boundary_conditions", "compute_residual")
 
-SOLVERS = ("jacobi", "rb_sor_gauss_seidel", "multigrid")
+SOLVERS = ("jacobi", "gauss_seidel", "rb_sor_gauss_seidel", "multigrid")
 
 
 class OracleImpl:

@@ -228,6 +228,24 @@ def test_rbsor_bitwise(hip, shape, iterations, omega):
     np.testing.assert_array_equal(host(out), ref)
 
 
+@pytest.mark.parametrize("shape", SHAPES + [(1, 17), (23, 1)])
+@pytest.mark.parametrize("iterations", [1, 2, 7])
+@pytest.mark.parametrize("kind", ["open", "obstacles"])
+def test_gauss_seidel_wavefront_bitwise(hip, shape, iterations, kind):
+    """The lexicographic sweep run as a wavefront equals the serial sweep."""
+    _, sol = hip
+    H, W = shape
+    _, fmu, fmv, _ = masks_for(H, W, kind, seed=5)
+    d = rhs_for(H, W, fmu, fmv, 14)
+    p0 = np.random.default_rng(4).random((H, W))
+    ref = p0.copy()
+    O.gauss_seidel_pressure_solver(ref, d, fmu, fmv, 1.0, 1.0 / max(H, 1), 0.005, iterations)
+    out = dev(p0)
+    sol.gauss_seidel_pressure_solver(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / max(H, 1), 0.005,
+                                     iterations)
+    np.testing.assert_array_equal(host(out), ref)
+
+
 def test_rbsor_tolerance_loop(hip):
     _, sol = hip
     H, W = 64, 64

@@ -69,12 +69,12 @@ def test_solver_registry_and_dispatch():
     from cfd_amd.solvers.params import MultigridParams
     from cfd_amd.solvers.registry import SOLVER_REGISTRY, get_solver, register_solver
 
-    assert {m for m, b in SOLVER_REGISTRY if b == "hip"} == {"jacobi", "rb_sor_gauss_seidel",
-                                                             "multigrid"}
+    assert {m for m, b in SOLVER_REGISTRY if b == "hip"} == {"jacobi", "gauss_seidel",
+                                                             "rb_sor_gauss_seidel", "multigrid"}
     with pytest.raises(ValueError, match="already registered"):
         register_solver("jacobi", "hip", lambda *a, **k: None)
     with pytest.raises(ValueError, match="Unknown pressure solver"):
-        get_solver("gauss_seidel", "hip")
+        get_solver("gauss_seidel", "cuda")
 
     seen = {}
 

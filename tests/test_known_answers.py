@@ -240,7 +240,7 @@ def test_pressure_solver_reduces_divergence(impl, small_grid_size, face_masks, f
     assert r1 < r0 * 0.1
 
 
-@pytest.mark.parametrize("method", ["jacobi", "rb_sor_gauss_seidel", "multigrid"])
+@pytest.mark.parametrize("method", ["jacobi", "gauss_seidel", "rb_sor_gauss_seidel", "multigrid"])
 def test_pressure_solver_zero_divergence_stays_zero(impl, small_grid_size, face_masks,
                                                     fluid_density, method):
     fmu, fmv = face_masks
@@ -252,7 +252,7 @@ def test_pressure_solver_zero_divergence_stays_zero(impl, small_grid_size, face_
 
 
 # ------------------------------------------------- pipeline invariants
-@pytest.mark.parametrize("method", ["jacobi", "rb_sor_gauss_seidel", "multigrid"])
+@pytest.mark.parametrize("method", ["jacobi", "gauss_seidel", "rb_sor_gauss_seidel", "multigrid"])
 def test_zero_velocity_zero_pressure_is_noop(impl, domain, method):
     H, W = domain["height"], domain["width"]
     fmu, fmv = domain["face_mask_u"], domain["face_mask_v"]
@@ -274,6 +274,7 @@ def test_zero_velocity_zero_pressure_is_noop(impl, domain, method):
 
 @pytest.mark.parametrize("method,kw", [
     ("jacobi", {"iterations": 40}),
+    ("gauss_seidel", {"iterations": 40}),
     ("rb_sor_gauss_seidel", {"iterations": 40}),
     ("multigrid", {"v_cycles": 2, "smoother_type": "jacobi", "num_levels": 4,
                    "smoother_iterations": 2, "coarsest_iterations": 30}),
