@@ -406,8 +406,10 @@ int smk_profile_read(smk_profile_record* out, int n, int reset);
 /* Red-black smoother tile variant per dtype for the fine level (which = 0:
  * finest multigrid level and the single-grid solvers) and the coarse levels
  * (which = 1): 0 = 64x48 LDS tile, 1 = 64x32 LDS tile, 2 = 64x64
- * register-resident region; which = 2 selects the threads of the
- * single-workgroup small-level V-cycle: 0 = 256, 1 = 512, 2 = 1024.
+ * register-resident region, 3 = 64x48 tile with only p in LDS (rhs and
+ * flags in registers), 4 = the same at 64x32, 5 = the same at 64x64 with
+ * 512 threads.  which = 2 selects the threads of the single-workgroup
+ * small-level V-cycle: 0 = 256, 1 = 512, 2 = 1024.
  * Results do not depend on the variant. */
 int smk_set_smoother_variant(int dtype, int which, int variant);
 int smk_get_smoother_variant(int dtype, int which);

@@ -301,6 +301,321 @@ __global__ __launch_bounds__(NT) void k_smooth_fine(const SmoothArgs<T> a) {
   smooth_tile64_body<T, MODE_RB, RH, NT>(a, bx, by, by * gridDim.x + bx);
 }
 
+// Red-black tile smoothing with only p (and the flags, for the restriction)
+// staged in LDS.  Lane x of wave w updates the cells of region column x in
+// the row pairs m = w + k*NW (k < K), one cell per pair and pass, so the
+// right-hand side and flags of those 2K cells stay in the lane's registers
+// for the whole launch, indexed by colour: the colour-c cell of pair m is
+// row 2m + ((c + x) & 1).  Half the LDS of smooth_tile64_body, so more
+// workgroups share a CU, and two fewer LDS reads per cell update.  The
+// per-cell arithmetic is the same expression in the same order.  Residuals
+// for the restriction are computed by the owning lane and passed through the
+// p array once the passes are done.
+__device__ __forceinline__ bool fbit(const uint32_t* fw, int k, uint32_t bit) {
+  return (fw[k >> 2] & (bit << (8 * (k & 3)))) != 0u;
+}
+
+// Per-tile state of a lane: the rhs and flags of its 2K cells by colour, and
+// the wave-uniform mask of pairs whose cells are fluid with all four faces
+// open in every lane (the stencil then needs no selects: a scalar branch).
+template <typename T, int RH, int NT>
+struct RTile {
+  static constexpr int RW = 64;
+  static constexpr int NW = NT / 64;
+  static_assert(RH % (2 * NW) == 0, "row pairs must split evenly over the waves");
+  static constexpr int K = RH / (2 * NW);  // row pairs per lane
+  static constexpr int KW = (K + 3) / 4;   // packed flag words per colour
+  T rb[2][K];          // rhs of the colour-c cell of pair k
+  uint32_t fw[2][KW];  // flags of the colour-c cell of pair k, byte k & 3 of word k >> 2
+  uint32_t open[2];
+};
+
+// Raw loads of the tile's rows for the lane's pairs h0 .. h0+KH-1 (clamped
+// into the grid, unconditional: a lane issues them back to back).
+template <typename T, int RH, int NT, int KH>
+__device__ __forceinline__ void rtile_issue(const SmoothArgs<T>& a, int bx, int by, int h0,
+                                            T (&pv)[KH][2], T (&rv)[KH][2],
+                                            uint32_t (&fv)[KH][2]) {
+  using RT = RTile<T, RH, NT>;
+  const int R = a.R;
+  const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const int gxc = gx < 0 ? 0 : (gx >= a.W ? a.W - 1 : gx);
+#pragma unroll
+  for (int k = 0; k < KH; ++k)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
+      const int gyc = gy < 0 ? 0 : (gy >= a.H ? a.H - 1 : gy);
+      const size_t g = (size_t)gyc * a.W + gxc;
+      fv[k][q] = a.flags[g];
+      rv[k][q] = a.rhs[g];
+      pv[k][q] = a.zero_init ? (T)0 : a.p_in[g];
+    }
+}
+
+// Apply the prolonged correction, zero what lies outside the grid, and place
+// p and the flags in LDS and the rhs and flags in the lane's registers.
+template <typename T, int RH, int NT, int KH>
+__device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int by, int h0,
+                                             T (&pv)[KH][2], const T (&rv)[KH][2],
+                                             const uint32_t (&fv)[KH][2], T* sp, uint8_t* sf,
+                                             RTile<T, RH, NT>& t) {
+  using RT = RTile<T, RH, NT>;
+  constexpr uint32_t F_ALL = F_FLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
+  const int R = a.R;
+  const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 1;
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const bool colin = gx >= 0 && gx < a.W;
+  if (a.coarse_e != nullptr) {
+#pragma unroll
+    for (int k = 0; k < KH; ++k)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
+        const T corr = (T)sample_field(a.coarse_e, a.Hc, a.Wc, (gx + 0.5) * 0.5 - 0.5,
+                                       (gy + 0.5) * 0.5 - 0.5);
+        pv[k][q] = pv[k][q] + ((fv[k][q] & F_FLUID) ? corr : (T)0);
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    uint32_t f[2];
+    T r[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ly = 2 * (wave + (h0 + k) * RT::NW) + q;
+      const int gy = gy0 + ly;
+      const bool in = colin && gy >= 0 && gy < a.H;
+      const int o = ly * RT::RW + lane;
+      f[q] = in ? fv[k][q] : 0u;
+      r[q] = in ? rv[k][q] : (T)0;
+      sp[o] = in ? pv[k][q] : (T)0;
+      sf[o] = (uint8_t)f[q];
+    }
+    // row 2m + q has colour (q + x) & 1
+    const int kk = h0 + k;
+    t.rb[0][kk] = px ? r[1] : r[0];
+    t.rb[1][kk] = px ? r[0] : r[1];
+    const uint32_t f0 = px ? f[1] : f[0], f1 = px ? f[0] : f[1];
+    t.fw[0][kk >> 2] |= f0 << (8 * (kk & 3));
+    t.fw[1][kk >> 2] |= f1 << (8 * (kk & 3));
+    if (__ballot((f0 & F_ALL) == F_ALL) == ~0ull) t.open[0] |= 1u << kk;
+    if (__ballot((f1 & F_ALL) == F_ALL) == ~0ull) t.open[1] |= 1u << kk;
+  }
+}
+
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_reset(RTile<T, RH, NT>& t) {
+  t.open[0] = t.open[1] = 0u;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int q = 0; q < RTile<T, RH, NT>::KW; ++q) t.fw[c][q] = 0u;
+}
+
+// The stencil of the lane's colour-c cell of pair k at LDS offset o; returns
+// the neighbour sum with closed faces replaced by the centre value.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ T rtile_nsum(const RTile<T, RH, NT>& t, int c, int k, const T* sp,
+                                        int o, T& ce, bool& fl) {
+  constexpr int RW = RTile<T, RH, NT>::RW;
+  ce = sp[o];
+  const T lv = sp[o - 1], rv = sp[o + 1], dv = sp[o - RW], uv = sp[o + RW];
+  if ((t.open[c] >> k) & 1u) {
+    fl = true;
+    return lv + rv + dv + uv;
+  }
+  const uint32_t* fwc = t.fw[c];
+  const T l = fbit(fwc, k, F_LEFT) ? lv : ce;
+  const T r = fbit(fwc, k, F_RIGHT) ? rv : ce;
+  const T d = fbit(fwc, k, F_DOWN) ? dv : ce;
+  const T u = fbit(fwc, k, F_UP) ? uv : ce;
+  fl = fbit(fwc, k, F_FLUID);
+  return l + r + d + u;
+}
+
+// One colour pass over the lane's cells of colour C (a compile-time colour
+// keeps every register array index constant).  Cells of one colour never
+// neighbour each other: every new value of the lane is computed before any
+// is stored.
+template <int C, typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_sweep(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                            T* sp, int lo) {
+  using RT = RTile<T, RH, NT>;
+  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int par = (lane + C) & 1;
+  const bool xin = lane >= lo && lane < RW - lo;
+  T nv[K];
+  int oo[K];
+  bool st[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int ly = 2 * (wave + k * NW) + par;
+    const bool valid = xin && ly >= lo && ly < RH - lo;
+    const int o = valid ? ly * RW + lane : RW + 1;
+    T ce;
+    bool fl;
+    const T ns = rtile_nsum(t, C, k, sp, o, ce, fl);
+    nv[k] = (T)relax_value(0, ce, ns, t.rb[C][k], a.alpha, a.omega);
+    oo[k] = o;
+    st[k] = valid && fl;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if (st[k]) sp[oo[k]] = nv[k];
+}
+
+// npass colour passes; the valid window shrinks by one cell per pass.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_passes(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                             T* sp) {
+  for (int pass = 0; pass < a.npass; ++pass) {
+    if (((a.first_colour + pass) & 1) == 0)
+      rtile_sweep<0>(a, t, sp, pass + 1);
+    else
+      rtile_sweep<1>(a, t, sp, pass + 1);
+    __syncthreads();
+  }
+}
+
+// Write the tile interior of p; then the residuals of the lane's own cells
+// in the output window feed the restriction (through the p array) and the
+// residual norm.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                               int bx, int by, int part, T* sp,
+                                               const uint8_t* sf, double* red) {
+  using RT = RTile<T, RH, NT>;
+  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
+  const int R = a.R;
+  const int TX = RW - 2 * R, TY = RH - 2 * R;
+  const int H = a.H, W = a.W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const bool colin = gx >= 0 && gx < W;
+  const bool xout = lane >= R && lane < R + TX && colin;
+  for (int ly = R + wave; ly < R + TY; ly += NW) {
+    const int gy = gy0 + ly;
+    if (xout && gy < H) a.p_out[(size_t)gy * W + gx] = sp[ly * RW + lane];
+  }
+  if (a.coarse_rhs == nullptr && a.norm_partial == nullptr) return;
+
+  T res[2][K];
+  double acc = 0.0;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int par = (lane + c) & 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int ly = 2 * (wave + k * NW) + par;
+      const bool win = xout && ly >= R && ly < R + TY && gy0 + ly < H;
+      const int o = win ? ly * RW + lane : RW + 1;
+      T ce;
+      bool fl;
+      const T ns = rtile_nsum(t, c, k, sp, o, ce, fl);
+      const double rr = residual_value(ce, ns, t.rb[c][k], a.alpha);
+      res[c][k] = (T)rr;
+      acc += (win && fl) ? rr * rr : 0.0;
+    }
+  }
+
+  if (a.coarse_rhs != nullptr) {
+    __syncthreads();  // every lane has read p
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int par = (lane + c) & 1;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int ly = 2 * (wave + k * NW) + par;
+        if (xout && ly >= R && ly < R + TY && gy0 + ly < H) sp[ly * RW + lane] = res[c][k];
+      }
+    }
+    __syncthreads();
+    const int CX = TX / 2, CY = TY / 2;
+    const int Wc = W / 2, Hc = H / 2;
+    const int cx = lane & 31;
+    for (int cyb = 2 * wave; cyb < CY; cyb += 2 * NW) {
+      const int cy = cyb + (lane >> 5);
+      if (cy >= CY || cx >= CX) continue;
+      const int Jc = by * CY + cy, Ic = bx * CX + cx;
+      if (Jc >= Hc || Ic >= Wc) continue;
+      // the children's residuals summed in float64 over the fluid children
+      // in the (dj, di) order of smooth_tile64_body
+      double total = 0.0;
+      int count = 0;
+#pragma unroll
+      for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+        for (int di = 0; di < 2; ++di) {
+          const int o = (R + 2 * cy + dj) * RW + R + 2 * cx + di;
+          const bool fl = sf[o] & F_FLUID;
+          const T rv = sp[o];
+          total += fl ? (double)rv : 0.0;
+          count += fl ? 1 : 0;
+        }
+      const T mean = count > 0 ? (T)(total / (double)count) : (T)0;
+      a.coarse_rhs[(size_t)Jc * Wc + Ic] = (T)((double)mean / a.coarse_div);
+    }
+  }
+
+  if (a.norm_partial != nullptr) {
+    const double tot = block_sum<NT>(acc, red);
+    if (threadIdx.x == 0) a.norm_partial[part] = tot;
+  }
+}
+
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int bx, int by,
+                                                    int part) {
+  using RT = RTile<T, RH, NT>;
+  __shared__ T sp[RH * RT::RW];
+  __shared__ uint8_t sf[RH * RT::RW];
+  __shared__ double red[RT::NW];
+  if (a.gate != nullptr && *a.gate == 0) return;
+  RT t;
+  rtile_reset(t);
+  // in two halves of the lane's pairs, to bound the registers of loads in
+  // flight
+  constexpr int K = RT::K;
+  constexpr int KH = (K % 2 == 0) ? K / 2 : K;
+#pragma unroll
+  for (int h0 = 0; h0 < K; h0 += KH) {
+    T pv[KH][2], rv[KH][2];
+    uint32_t fv[KH][2];
+    rtile_issue<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv);
+    rtile_commit<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv, sp, sf, t);
+  }
+  t.open[0] = __builtin_amdgcn_readfirstlane(t.open[0]);
+  t.open[1] = __builtin_amdgcn_readfirstlane(t.open[1]);
+  __syncthreads();
+  rtile_passes(a, t, sp);
+  rtile_epilogue(a, t, bx, by, part, sp, sf, red);
+}
+
+// waves per SIMD the register allocation must allow: 4 when a float64 lane
+// holds six or more row pairs (more would spill), 5 otherwise
+template <typename T, int RH, int NT>
+constexpr int r_waves() { return (sizeof(T) == 8 && RH / (2 * (NT / 64)) >= 6) ? 4 : 5; }
+
+template <typename T, int RH, int NT>
+__global__ __launch_bounds__(NT, (r_waves<T, RH, NT>())) void k_smooth_fine_r(const SmoothArgs<T> a) {
+  int bx, by;
+  xcd_tile(bx, by);
+  smooth_tile64r_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
+}
+
+template <typename T, int RH, int NT>
+__global__ __launch_bounds__(NT, (r_waves<T, RH, NT>())) void k_smooth_tile64_r(const SmoothArgs<T> a) {
+  int bx, by;
+  xcd_tile(bx, by);
+  smooth_tile64r_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
+}
+
 // Red-black smoothing with the region in registers.  A workgroup of NW
 // wavefronts stages a 64 x (NW*BR) region; lane x owns region column x, wave
 // w owns rows [w*BR, (w+1)*BR).  Rows are held in pairs (2m, 2m+1) as
@@ -329,13 +644,14 @@ __device__ __forceinline__ T rb_relax(T ce, T l, T r, T d, T u, T b, uint32_t f,
 }
 
 template <typename T>
-__device__ __forceinline__ T rb_resid(T ce, T l, T r, T d, T u, T b, uint32_t f, double alpha) {
+__device__ __forceinline__ double rb_resid(T ce, T l, T r, T d, T u, T b, uint32_t f,
+                                           double alpha) {
   const T lv = (f & F_LEFT) ? l : ce;
   const T rv = (f & F_RIGHT) ? r : ce;
   const T dv = (f & F_DOWN) ? d : ce;
   const T uv = (f & F_UP) ? u : ce;
   const T ns = lv + rv + dv + uv;
-  return (T)residual_value(ce, ns, b, alpha);
+  return residual_value(ce, ns, b, alpha);
 }
 
 template <typename T, int NW, int BR>
@@ -486,12 +802,15 @@ __global__ __launch_bounds__(NW * 64) void k_rb_reg(const SmoothArgs<T> a) {
     const T bu = m < P - 1 ? B[m + 1] : b_above;
     const T ad = m > 0 ? A[m - 1] : a_below;
     const T au = m < P - 1 ? A[m + 1] : a_above;
-    const T rA = rb_resid(A[m], from_left(B[m]), from_right(B[m]), px ? B[m] : bd,
-                          px ? bu : B[m], RA[m], FA[m], a.alpha);
-    const T rB = rb_resid(B[m], from_left(A[m]), from_right(A[m]), px ? ad : A[m],
-                          px ? A[m] : au, RB[m], FB[m], a.alpha);
-    const T rlo = px ? rB : rA;  // row 2m
-    const T rhi = px ? rA : rB;  // row 2m+1
+    // residuals in float64; the restriction takes them in the level's dtype
+    // (as stored by the reference), the norm in float64
+    const double dA = rb_resid(A[m], from_left(B[m]), from_right(B[m]), px ? B[m] : bd,
+                               px ? bu : B[m], RA[m], FA[m], a.alpha);
+    const double dB = rb_resid(B[m], from_left(A[m]), from_right(A[m]), px ? ad : A[m],
+                               px ? A[m] : au, RB[m], FB[m], a.alpha);
+    const double dlo = px ? dB : dA;  // row 2m
+    const double dhi = px ? dA : dB;  // row 2m+1
+    const T rlo = (T)dlo, rhi = (T)dhi;
     const uint32_t flo = px ? FB[m] : FA[m];
     const uint32_t fhi = px ? FA[m] : FB[m];
     const int ly = r0 + 2 * m;
@@ -519,10 +838,13 @@ __global__ __launch_bounds__(NW * 64) void k_rb_reg(const SmoothArgs<T> a) {
       }
     }
     if (a.norm_partial != nullptr) {
+      // each row of the pair against the output window on its own: with an
+      // odd halo a pair straddles the window's edge
+      const bool yin1 = ly + 1 >= R && ly + 1 < R + TY;
       const bool in0 = xout && yin && gy < H && (flo & F_FLUID);
-      const bool in1 = xout && yin && gy + 1 < H && (fhi & F_FLUID);
-      acc += in0 ? (double)rlo * (double)rlo : 0.0;
-      acc += in1 ? (double)rhi * (double)rhi : 0.0;
+      const bool in1 = xout && yin1 && gy + 1 < H && (fhi & F_FLUID);
+      acc += in0 ? dlo * dlo : 0.0;
+      acc += in1 ? dhi * dhi : 0.0;
     }
   }
   if (a.norm_partial != nullptr) {
@@ -639,9 +961,114 @@ __device__ __forceinline__ void small_pass(const T* cur, T* dst, const T* rhs,
   }
 }
 
+// Red-black passes over a level of at most 64 x 64 cells with the cells in
+// registers (the layout of k_rb_reg): lane x owns column x, wave w owns the
+// row pairs [w*P, (w+1)*P); A[m] / B[m] are the colour-0 / colour-1 cell of
+// pair m.  Horizontal neighbours come from lanes x -/+ 1 (__shfl), vertical
+// ones from the lane's own pairs, and across band edges from the level's LDS
+// array: a pass first stores the cells the previous pass updated (the colour
+// this pass reads), so after one barrier the band-edge rows are current.  A
+// pass reads one colour's cells and the next writes the other colour, so a
+// single barrier per pass separates every store from the reads it could
+// disturb.  Cells outside the level are never updated and never used: the
+// flags of the level's border cells have those neighbour bits cleared.
+// The per-cell arithmetic is small_pass's, in the same order.
+template <typename T, int NT, int PMAX>
+__device__ __forceinline__ void small_smooth_rb_reg(T* p, const T* rhs, const uint8_t* fl, int H,
+                                                    int W, double alpha, double omega,
+                                                    int npass) {
+  constexpr int NW = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 1;
+  constexpr int P = PMAX;        // ceil(ceil(H / 2) / NW): the caller dispatches on it
+  const int r0 = 2 * wave * P;  // first row of this wave's band
+  const bool lin = lane < W;
+  T A[PMAX], B[PMAX], RA[PMAX], RB[PMAX];
+  uint32_t FA[PMAX], FB[PMAX];
+#pragma unroll
+  for (int m = 0; m < PMAX; ++m) {
+    const int ya = r0 + 2 * m + px, yb = r0 + 2 * m + 1 - px;
+    const bool oka = lin && m < P && ya < H, okb = lin && m < P && yb < H;
+    const int oa = oka ? ya * W + lane : 0, ob = okb ? yb * W + lane : 0;
+    A[m] = oka ? p[oa] : (T)0;
+    B[m] = okb ? p[ob] : (T)0;
+    RA[m] = oka ? rhs[oa] : (T)0;
+    RB[m] = okb ? rhs[ob] : (T)0;
+    FA[m] = oka ? fl[oa] : 0u;
+    FB[m] = okb ? fl[ob] : 0u;
+  }
+  const int yb_lo = r0 - 1, yb_hi = r0 + 2 * P;
+  const bool has_lo = lin && yb_lo >= 0 && yb_lo < H, has_hi = lin && yb_hi < H;
+  for (int pass = 0; pass < npass; ++pass) {
+    const int colour = pass & 1;
+    if (pass > 0) {  // publish the cells the previous pass updated
+#pragma unroll
+      for (int m = 0; m < PMAX; ++m) {
+        const int yo = colour == 0 ? r0 + 2 * m + 1 - px : r0 + 2 * m + px;
+        if (lin && m < P && yo < H) p[yo * W + lane] = colour == 0 ? B[m] : A[m];
+      }
+    }
+    __syncthreads();
+    const T below = has_lo ? p[yb_lo * W + lane] : (T)0;
+    const T above = has_hi ? p[yb_hi * W + lane] : (T)0;
+    if (colour == 0) {
+#pragma unroll
+      for (int m = 0; m < PMAX; ++m) {
+        const T bd = m > 0 ? B[m - 1] : below;
+        const T bu = m + 1 < P ? B[m + 1 < PMAX ? m + 1 : m] : above;
+        const T l = from_left(B[m]), r = from_right(B[m]);
+        const T d = px ? B[m] : bd;
+        const T u = px ? bu : B[m];
+        const bool ok = (FA[m] & F_FLUID) != 0u;
+        const T nv = rb_relax(A[m], l, r, d, u, RA[m], FA[m], alpha, omega);
+        A[m] = ok ? nv : A[m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < PMAX; ++m) {
+        const T ad = m > 0 ? A[m - 1] : below;
+        const T au = m + 1 < P ? A[m + 1 < PMAX ? m + 1 : m] : above;
+        const T l = from_left(A[m]), r = from_right(A[m]);
+        const T d = px ? ad : A[m];
+        const T u = px ? A[m] : au;
+        const bool ok = (FB[m] & F_FLUID) != 0u;
+        const T nv = rb_relax(B[m], l, r, d, u, RB[m], FB[m], alpha, omega);
+        B[m] = ok ? nv : B[m];
+      }
+    }
+  }
+  // store the colour the last pass updated; the other one is already there
+  if (npass > 0) {
+    const int last = (npass - 1) & 1;
+#pragma unroll
+    for (int m = 0; m < PMAX; ++m) {
+      const int yo = last == 0 ? r0 + 2 * m + px : r0 + 2 * m + 1 - px;
+      if (lin && m < P && yo < H) p[yo * W + lane] = last == 0 ? A[m] : B[m];
+    }
+  }
+  __syncthreads();
+}
+
 template <typename T, int MODE, int NT>
 __device__ __forceinline__ void small_smooth(T* p, T* tmp, const T* rhs, const uint8_t* fl, int H, int W,
                              double alpha, double omega, int npass) {
+  if (MODE == MODE_RB && W <= 64 && H <= 64) {
+    // pairs per wave as a compile-time count, so no pass runs predicated-off
+    // work for pairs the level does not have
+    constexpr int NW = NT / 64;
+    const int P = (((H + 1) >> 1) + NW - 1) / NW;
+    switch (P) {
+      case 1: small_smooth_rb_reg<T, NT, 1>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      case 2: small_smooth_rb_reg<T, NT, 2>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      case 3: small_smooth_rb_reg<T, NT, 3>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      case 4: small_smooth_rb_reg<T, NT, 4>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      case 5: small_smooth_rb_reg<T, NT, 5>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      case 6: small_smooth_rb_reg<T, NT, 6>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      case 7: small_smooth_rb_reg<T, NT, 7>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      case 8: small_smooth_rb_reg<T, NT, 8>(p, rhs, fl, H, W, alpha, omega, npass); return;
+      default: break;  // more than 8 pairs per wave: the LDS passes below
+    }
+  }
   const int N = H * W;
   T* cur = p;
   for (int pass = 0; pass < npass; ++pass) {
@@ -983,12 +1410,20 @@ __global__ void k_state_init(SolveState* __restrict__ st) {
 // relative residual against ||alpha div||^2 = *b2, and the decision the host
 // loops of the solvers take (stop at rel <= tol, or after `cap` cycles --
 // `inner` V-cycles per check).
+//
+// The resulting state is also written to *host_out (a slot of a pinned host
+// ring, may be null), so the host learns the outcome without a copy of its
+// own on the stream.
 __global__ __launch_bounds__(1024) void k_check(const double* __restrict__ partial, int n,
                                                 const double* __restrict__ b2,
                                                 SolveState* __restrict__ st, double tol,
-                                                int cap, int inner) {
+                                                int cap, int inner,
+                                                SolveState* __restrict__ host_out) {
   __shared__ double red[1024 / 64];
-  if (st->active == 0) return;
+  if (st->active == 0) {
+    if (threadIdx.x == 0 && host_out != nullptr) *host_out = *st;
+    return;
+  }
   double acc = 0.0;
   for (int k = threadIdx.x; k < n; k += 1024) acc += partial[k];
   const double tot = block_sum<1024>(acc, red);
@@ -998,10 +1433,9 @@ __global__ __launch_bounds__(1024) void k_check(const double* __restrict__ parti
   st->rel = rel;
   const int c = st->cycles;
   const bool done = (tol > 0.0 ? rel <= tol : c >= cap) || c * inner >= cap;
-  if (done)
-    st->active = 0;
-  else
-    st->cycles = c + 1;
+  SolveState out{done ? 0 : 1, done ? c : c + 1, rel};
+  *st = out;
+  if (host_out != nullptr) *host_out = out;
 }
 
 __global__ void k_solid_from_flags(const uint8_t* __restrict__ flags, long n,
@@ -1081,14 +1515,32 @@ constexpr int JAC_RH = 32;
 // Tile variants of the smoothing launch.  The fine level (HBM-bound) and the
 // coarse levels (latency-bound: few tiles) are tuned separately, and the fine
 // level gets its own kernel symbol so profiles separate the two.
-enum : int { VAR_LDS48 = 0, VAR_LDS32 = 1, VAR_REG64 = 2, NUM_VARIANTS = 3 };
-static int g_variant[2][2] = {{VAR_LDS48, VAR_LDS32}, {VAR_LDS48, VAR_LDS32}};  // [f32?][coarse?]
+enum : int {
+  VAR_LDS48 = 0,   // 64 x 48 region, p / rhs / flags in LDS
+  VAR_LDS32 = 1,   // 64 x 32 region, p / rhs / flags in LDS
+  VAR_REG64 = 2,   // 64 x 64 region in registers (k_rb_reg)
+  VAR_LDS48R = 3,  // 64 x 48 region, p in LDS, rhs / flags in registers, 256 threads
+  VAR_LDS32R = 4,  // 64 x 32 region, as VAR_LDS48R
+  VAR_LDS64R = 5,  // 64 x 64 region, as VAR_LDS48R with 512 threads
+  NUM_VARIANTS = 6
+};
+// Defaults per dtype [f32?][coarse?], from timing every variant on the 2048^2
+// V(2,2) workload: float64 finest level 64 x 32 with the rhs in registers
+// (37 us vs 41 us for 64 x 48 in LDS), float32 finest 64 x 64 with the rhs
+// in registers, coarse levels 64 x 32 in LDS.
+static int g_variant[2][2] = {{VAR_LDS32R, VAR_LDS32}, {VAR_LDS64R, VAR_LDS32}};
 
 // Threads of the single-workgroup small-level V-cycle, per dtype [f32?].
-static int g_small_nt[2] = {512, 512};
+static int g_small_nt[2] = {1024, 1024};
 constexpr int SMALL_NT_CHOICES[3] = {256, 512, 1024};
 
-inline int variant_rh(int v) { return v == VAR_LDS48 ? 48 : (v == VAR_LDS32 ? 32 : RB_RH); }
+inline int variant_rh(int v) {
+  switch (v) {
+    case VAR_LDS48: case VAR_LDS48R: return 48;
+    case VAR_LDS32: case VAR_LDS32R: return 32;
+    default: return RB_RH;  // VAR_REG64, VAR_LDS64R
+  }
+}
 
 template <typename T>
 int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = false) {
@@ -1115,6 +1567,20 @@ int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = fa
     dim3 grid(cdiv(a.W, tile_tx(a.R)), cdiv(a.H, rh - 2 * a.R));
     if (v == VAR_REG64)
       hipLaunchKernelGGL((k_rb_reg<T, RB_NW, RB_BR>), grid, dim3(RB_NW * 64), 0, s, a);
+#define SMK_R_LAUNCH(RHV, NTV)                                                             \
+  do {                                                                                     \
+    if (fine)                                                                              \
+      hipLaunchKernelGGL((k_smooth_fine_r<T, RHV, NTV>), grid, dim3(NTV), 0, s, a);        \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_smooth_tile64_r<T, RHV, NTV>), grid, dim3(NTV), 0, s, a);      \
+  } while (0)
+    else if (v == VAR_LDS48R)
+      SMK_R_LAUNCH(48, 256);
+    else if (v == VAR_LDS32R)
+      SMK_R_LAUNCH(32, 256);
+    else if (v == VAR_LDS64R)
+      SMK_R_LAUNCH(64, 512);
+#undef SMK_R_LAUNCH
     else if (v == VAR_LDS32)
       hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, 32, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
     else if (fine)
@@ -1403,25 +1869,24 @@ inline double optimal_omega(int size) {
   return w < 1.95 ? w : 1.95;
 }
 
-// Queue gated iterations -- iter(k) enqueues iteration k, ending with a
-// k_check on d_state -- until the device reports the solve done.  The
-// outcome of iteration k-1 is read (from a pinned ring slot) while iteration
+// Queue gated iterations -- iter(k, slot) enqueues iteration k, ending with a
+// k_check on d_state that also writes the state to `slot`, the device
+// address of pinned ring slot k % 4 -- until the device reports the solve
+// done.  The outcome of iteration k-1 is read from its slot while iteration
 // k is already queued, so the GPU never waits for the host; the one
 // iteration queued after the last useful one is gated off and does nothing.
 template <typename F>
-int run_gated_ring(SolveState* d_state, SolveState* h_ring, hipEvent_t* ev, int max_iter,
-                   F&& iter, SolveState* fin, hipStream_t s) {
+int run_gated_ring(SolveState* d_state, SolveState* h_ring, SolveState* d_ring, hipEvent_t* ev,
+                   int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
   for (int k = 0; k <= max_iter; ++k) {
-    int rc = iter(k);
-    if (rc) return rc;
     const int slot = k & 3;
-    SMK_CHECK_HIP(hipMemcpyAsync(&h_ring[slot], d_state, sizeof(SolveState),
-                                 hipMemcpyDeviceToHost, s));
+    int rc = iter(k, d_ring + slot);
+    if (rc) return rc;
     SMK_CHECK_HIP(hipEventRecord(ev[slot], s));
     if (k >= 1) {
       const int prev = (k - 1) & 3;
       SMK_CHECK_HIP(hipEventSynchronize(ev[prev]));
-      if (h_ring[prev].active == 0) break;
+      if (reinterpret_cast<volatile SolveState*>(h_ring)[prev].active == 0) break;
     }
   }
   SMK_CHECK_HIP(hipStreamSynchronize(s));
@@ -1433,6 +1898,7 @@ int run_gated_ring(SolveState* d_state, SolveState* h_ring, hipEvent_t* ev, int 
 // caller workspace, not a handle): one set per host thread, kept until exit.
 struct GateRing {
   SolveState* h_ring = nullptr;
+  SolveState* d_ring = nullptr;  // device address of h_ring
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
@@ -1440,6 +1906,7 @@ static int thread_gate_ring(GateRing** out) {
   thread_local GateRing g;
   if (g.h_ring == nullptr) {
     SMK_CHECK_HIP(hipHostMalloc((void**)&g.h_ring, 4 * sizeof(SolveState), hipHostMallocDefault));
+    SMK_CHECK_HIP(hipHostGetDevicePointer((void**)&g.d_ring, g.h_ring, 0));
     for (hipEvent_t& e : g.ev) SMK_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   *out = &g;
@@ -1460,7 +1927,8 @@ int sweeps_gated(SolverWs& w, int iterations, int check_every, double tol, F&& c
   rc = check_launch("k_state_init");
   if (rc) return rc;
   SolveState fin{};
-  rc = run_gated_ring(w.state, g->h_ring, g->ev, nchunks + 1, [&](int k) -> int {
+  rc = run_gated_ring(w.state, g->h_ring, g->d_ring, g->ev, nchunks + 1,
+                      [&](int k, SolveState* slot) -> int {
     int r = SMK_OK;
     if (k > 0) {
       const int start = (k - 1) * check_every;
@@ -1471,7 +1939,7 @@ int sweeps_gated(SolverWs& w, int iterations, int check_every, double tol, F&& c
     r = chunk(0, &w.state->active);  // the residual partials
     if (r) return r;
     hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, w.partial, nparts, b2, w.state, tol,
-                       nchunks, 1);
+                       nchunks, 1, slot);
     return check_launch("k_check");
   }, &fin, s);
   if (rc) return rc;
@@ -1680,10 +2148,11 @@ struct smk_mg {
   double* partial;
   double* result;
   int npartial;
-  // device-side convergence (gated iterations); host copies land in a
-  // pinned ring, one slot per queued iteration
+  // device-side convergence (gated iterations); each check also writes the
+  // state to a pinned ring, one slot per queued iteration
   smk::SolveState* d_state;
   smk::SolveState* h_ring;
+  smk::SolveState* d_ring;  // device address of h_ring
   hipEvent_t ev[4];
   int small_from[2];  // first level handled by k_vcycle_small: [native dtype, float32]
   int small_mode[2];  // smoother mode small_from was computed for
@@ -1910,7 +2379,7 @@ int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, do
 // iteration queued after the last useful one is gated off and does nothing.
 template <typename F>
 int run_gated(smk_mg* mg, int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
-  return run_gated_ring(mg->d_state, mg->h_ring, mg->ev, max_iter, iter, fin, s);
+  return run_gated_ring(mg->d_state, mg->h_ring, mg->d_ring, mg->ev, max_iter, iter, fin, s);
 }
 
 // Gated iterations need the finest level's ping-pong flips per V-cycle to be
@@ -1948,7 +2417,7 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
     rc = check_launch("k_state_init");
     if (rc) return rc;
     SolveState fin{};
-    rc = run_gated(mg, prm->v_cycles + 1, [&](int k) -> int {
+    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, SolveState* slot) -> int {
       int nparts = -1;
       int r = SMK_OK;
       if (k > 0) r = mg_vcycle<T>(mg, 0, false, grun, h, false, mg->partial, &nparts, s);
@@ -1959,7 +2428,7 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
       }
       if (r) return r;
       hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
-                         mg->result + 1, mg->d_state, prm->tolerance, prm->v_cycles, 1);
+                         mg->result + 1, mg->d_state, prm->tolerance, prm->v_cycles, 1, slot);
       return check_launch("k_check");
     }, &fin, s);
     if (rc) return rc;
@@ -2051,7 +2520,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     if (rc) return rc;
     const int nparts = norm_blocks(H, W);
     SolveState fin{};
-    rc = run_gated(mg, prm->v_cycles + 1, [&](int k) -> int {
+    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, SolveState* slot) -> int {
       int r = SMK_OK;
       if (k > 0) {
         for (int c = 0; c < inner && !r; ++c)
@@ -2069,7 +2538,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
                                         reinterpret_cast<float*>(L0.rhs32), grun.gate, s);
       if (r) return r;
       hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
-                         mg->result + 1, mg->d_state, tol, prm->v_cycles, inner);
+                         mg->result + 1, mg->d_state, tol, prm->v_cycles, inner, slot);
       return check_launch("k_check");
     }, &fin, s);
     if (rc) return rc;
@@ -2349,6 +2818,7 @@ int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W, int num_
   if (!rc) rc = mg_alloc((void**)&mg->d_state, sizeof(SolveState));
   if (!rc) {
     hipError_t e = hipHostMalloc((void**)&mg->h_ring, 4 * sizeof(SolveState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&mg->d_ring, mg->h_ring, 0);
     for (int k = 0; k < 4 && e == hipSuccess; ++k)
       e = hipEventCreateWithFlags(&mg->ev[k], hipEventDisableTiming);
     if (e != hipSuccess) rc = fail((int)e, "smk_mg_create: %s", hipGetErrorString(e));

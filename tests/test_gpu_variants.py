@@ -1,0 +1,138 @@
+"""Every red-black smoother tile variant (finest level and coarse levels) and
+every small-level workgroup size gives the same pressure, bit for bit: the
+float64 solves equal the oracle, the float32 solves equal the default
+variant's.  The variants differ only in how a tile is staged (LDS, registers)
+and in its size; the per-cell arithmetic is one expression."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from cfd_amd.backends.hip import _lib
+
+pytestmark = pytest.mark.gpu
+
+TILE_VARIANTS = sorted(_lib.SMOOTHER_VARIANTS)
+SMALL_VARIANTS = sorted(_lib.SMALL_VARIANTS)
+H, W = 200, 136
+
+
+def problem():
+    rng = np.random.default_rng(21)
+    solid = np.zeros((H, W), dtype=bool)
+    solid[0, :] = solid[-1, :] = True
+    solid[:, 0] = solid[:, -1] = True
+    solid |= rng.random((H, W)) < 0.08
+    fmu = np.zeros((H, W + 1), dtype=bool)
+    fmv = np.zeros((H + 1, W), dtype=bool)
+    fmu[:, 1:-1] = ~(solid[:, :-1] | solid[:, 1:])
+    fmv[1:-1, :] = ~(solid[:-1, :] | solid[1:, :])
+    u = rng.standard_normal((H, W + 1))
+    v = rng.standard_normal((H + 1, W))
+    d = np.zeros((H, W))
+    O.compute_divergence(u, v, d, fmu, fmv, 1.0 / H)
+    return d, fmu, fmv
+
+
+MG = dict(v_cycles=3, smoother_type="rb_sor_gauss_seidel", smoother_omega=1.0, num_levels=5,
+          smoother_iterations=2, coarsest_iterations=20, coarse_scale="rediscretized")
+
+
+def run_mg(dtype, **extra):
+    from cfd_amd.backends.hip import solvers
+
+    d, fmu, fmv = problem()
+    t = torch.float64 if dtype == "float64" else torch.float32
+    out = torch.zeros((H, W), dtype=t, device="cuda")
+    dd = torch.from_numpy(d).to(device="cuda", dtype=t)
+    mg = solvers.MultigridSolver()
+    mg(out, dd, torch.from_numpy(fmu).cuda(), torch.from_numpy(fmv).cuda(), 1.0, 1.0 / H, 0.005,
+       **dict(MG, **extra))
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), dict(solvers.LAST_SOLVE_INFO)
+
+
+def run_rbsor(dtype):
+    from cfd_amd.backends.hip import solvers
+
+    d, fmu, fmv = problem()
+    t = torch.float64 if dtype == "float64" else torch.float32
+    out = torch.zeros((H, W), dtype=t, device="cuda")
+    solvers.red_black_sor_gauss_seidel_pressure_solver(
+        out, torch.from_numpy(d).to(device="cuda", dtype=t), torch.from_numpy(fmu).cuda(),
+        torch.from_numpy(fmv).cuda(), 1.0, 1.0 / H, 0.005, 23, 1.7)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+class variant:
+    """Select one variant for the duration of a block, then restore."""
+
+    def __init__(self, dtype, which, name):
+        self.code = _lib.SMK_F64 if dtype == "float64" else _lib.SMK_F32
+        self.which, self.name = which, name
+
+    def __enter__(self):
+        L = _lib.load()
+        self.prev = L.smk_get_smoother_variant(self.code, _lib._WHICH[self.which])
+        _lib.set_smoother_variant(self.code, self.which, self.name)
+
+    def __exit__(self, *exc):
+        _lib.call("smk_set_smoother_variant", self.code, _lib._WHICH[self.which], self.prev)
+
+
+def oracle_mg():
+    d, fmu, fmv = problem()
+    ref = np.zeros((H, W))
+    O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **MG)
+    return ref
+
+
+def oracle_rbsor():
+    d, fmu, fmv = problem()
+    ref = np.zeros((H, W))
+    O.red_black_sor_gauss_seidel_pressure_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, 23, 1.7)
+    return ref
+
+
+@pytest.mark.parametrize("which", ["fine", "coarse"])
+@pytest.mark.parametrize("name", TILE_VARIANTS)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_tile_variant_multigrid(dtype, which, name):
+    want = oracle_mg() if dtype == "float64" else run_mg(dtype)[0]
+    with variant(dtype, which, name):
+        got, _ = run_mg(dtype)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("name", TILE_VARIANTS)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_tile_variant_rbsor(dtype, name):
+    want = oracle_rbsor() if dtype == "float64" else run_rbsor(dtype)
+    with variant(dtype, "fine", name):
+        got = run_rbsor(dtype)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("name", SMALL_VARIANTS)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_small_variant_multigrid(dtype, name):
+    want = oracle_mg() if dtype == "float64" else run_mg(dtype)[0]
+    with variant(dtype, "small", name):
+        got, _ = run_mg(dtype)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("name", TILE_VARIANTS)
+def test_tile_variant_tolerance_solve(name):
+    """The fused residual norm (post-smoothing of the finest level) drives a
+    tolerance solve to the oracle's cycle count under every variant."""
+    d, fmu, fmv = problem()
+    kw = dict(MG, v_cycles=60, tolerance=1e-9)
+    ref = np.zeros((H, W))
+    n_ref, rel_ref = O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **kw)
+    with variant("float64", "fine", name), variant("float64", "coarse", name):
+        got, info = run_mg("float64", v_cycles=60, tolerance=1e-9)
+    assert info["iterations"] == n_ref
+    assert info["rel_residual"] == pytest.approx(rel_ref, rel=1e-9)
+    np.testing.assert_array_equal(got, ref)
