@@ -102,13 +102,14 @@ def calibrate_copy(nbytes: int = 1 << 30) -> float:
     return 2 * nbytes / (e0.elapsed_time(e1) * 1e6)
 
 
-def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3):
+def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3,
+                inner: int = 1):
     import torch
 
     from cfd_amd import scenes
 
     sim = scenes.make_simulation(scene, dtype="float64", device="cuda",
-                                 solver_overrides={"precision": precision, "inner_cycles": 1})
+                                 solver_overrides={"precision": precision, "inner_cycles": inner})
     torch.cuda.synchronize()
     for _ in range(warmup):
         sim.substep(scene.dt)
@@ -378,6 +379,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", default="fp64,mixed")
+    ap.add_argument("--mixed-inner", type=int, default=2,
+                    help="FP32 V-cycles per FP64 refinement in the mixed variant")
     ap.add_argument("--batched-scenes", type=int, default=96,
                     help="scenes of the uneven-batch workload (0 skips it)")
     ap.add_argument("--diff-steps", type=int, default=64,
@@ -417,8 +420,10 @@ def main():
     results = {}
     for var in args.variants.split(","):
         log(f"variant {var}: warmup {args.warmup}, steps {args.steps}")
+        inner = args.mixed_inner if var == "mixed" else 1
         r, _ = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
-                           args.warmup)
+                           args.warmup, inner=inner)
+        r["inner_cycles_per_refinement"] = inner
         r["alg_bytes_per_cell_step"] = algorithmic_bytes_per_cell(
             r["v_cycles_per_step"], var == "mixed", r["levels"] or 8, r["refinements_per_step"])
         r["alg_GBps_whole_step"] = r["alg_bytes_per_cell_step"] * r["cell_steps_per_s"] / 1e9

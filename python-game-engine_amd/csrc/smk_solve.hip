@@ -1352,6 +1352,49 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
   if (threadIdx.x == 0) partial[by * gridDim.x + bx] = tot;
 }
 
+// The FP64 refinement step of the mixed solve in one pass: p_out = p_in + e
+// on every cell (k_add_correction), then the defect of p_out exactly as
+// k_residual_norm computes it, with every corrected value formed as
+// p_in + e in the same rounding.  p_out must not alias p_in (neighbours read
+// p_in while other threads write p_out).
+template <typename T>
+__global__ __launch_bounds__(NB_T) void k_correct_defect(const T* __restrict__ p_in,
+                                                         const float* __restrict__ e,
+                                                         T* __restrict__ p_out,
+                                                         const T* __restrict__ rhs,
+                                                         const uint8_t* __restrict__ flags,
+                                                         int H, int W, double alpha,
+                                                         double* __restrict__ partial,
+                                                         float* __restrict__ defect,
+                                                         const int* __restrict__ gate) {
+  __shared__ double red[NB_T / 64];
+  if (gate != nullptr && *gate == 0) return;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i = bx * NB_X + (threadIdx.x & (NB_X - 1));
+  const int j = by * NB_Y + (threadIdx.x / NB_X);
+  double acc = 0.0;
+  if (i < W && j < H) {
+    const size_t o = (size_t)j * W + i;
+    auto q = [&](size_t k) -> T { return p_in[k] + (T)e[k]; };
+    const uint8_t f = flags[o];
+    const T ce = q(o);
+    p_out[o] = ce;
+    double rv = 0.0;
+    if (f & F_FLUID) {
+      const T l = (f & F_LEFT) ? q(o - 1) : ce;
+      const T rr = (f & F_RIGHT) ? q(o + 1) : ce;
+      const T d = (f & F_DOWN) ? q(o - W) : ce;
+      const T u = (f & F_UP) ? q(o + W) : ce;
+      rv = poisson_residual(l, rr, d, u, ce, rhs[o], alpha);
+      acc = rv * rv;
+    }
+    defect[o] = (f & F_FLUID) ? (float)(rv / alpha) : 0.0f;
+  }
+  const double tot = block_sum<NB_T>(acc, red);
+  if (threadIdx.x == 0) partial[by * gridDim.x + bx] = tot;
+}
+
 // ||alpha*rhs||^2 partials over fluid cells (the p = 0 residual).
 template <typename T>
 __global__ __launch_bounds__(NB_T) void k_rhs_norm(const T* __restrict__ rhs,
@@ -2519,23 +2562,29 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     rc = check_launch("k_state_init");
     if (rc) return rc;
     const int nparts = norm_blocks(H, W);
+    // refinement k reads p from buf[(k-1) & 1] and writes buf[k & 1] (the
+    // correction and the defect in one pass); refinements gated off write
+    // nothing, so the result is in buf[executed refinements & 1]
+    double* buf[2] = {p, reinterpret_cast<double*>(L0.p[1])};
     SolveState fin{};
     rc = run_gated(mg, prm->v_cycles + 1, [&](int k, SolveState* slot) -> int {
       int r = SMK_OK;
+      float* defect = reinterpret_cast<float*>(L0.rhs32);
       if (k > 0) {
         for (int c = 0; c < inner && !r; ++c)
           r = mg_vcycle<float>(mg, 0, true, grun, h, c == 0, nullptr, nullptr, s);
         if (r) return r;
         const float* e = reinterpret_cast<const float*>(L0.p32[L0.cur32]);
         hipEvent_t ev = prof_begin(s);
-        hipLaunchKernelGGL(k_add_correction<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, e,
-                           n, grun.gate);
-        prof_end(ev, "correct_fp64", 20.0 * (double)n, s);
-        r = check_launch("k_add_correction");
-        if (r) return r;
+        hipLaunchKernelGGL(k_correct_defect<double>, dim3(cdiv(W, NB_X), cdiv(H, NB_Y)),
+                           dim3(NB_T), 0, s, buf[(k - 1) & 1], e, buf[k & 1], div, L0.flags, H,
+                           W, alpha, mg->partial, defect, grun.gate);
+        prof_end(ev, "correct_defect_fp64", 32.0 * (double)n, s);
+        r = check_launch("k_correct_defect");
+      } else {
+        r = residual_partials_dev<double>(p, div, L0.flags, H, W, alpha, mg->partial, defect,
+                                          grun.gate, s);
       }
-      r = residual_partials_dev<double>(p, div, L0.flags, H, W, alpha, mg->partial,
-                                        reinterpret_cast<float*>(L0.rhs32), grun.gate, s);
       if (r) return r;
       hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
                          mg->result + 1, mg->d_state, tol, prm->v_cycles, inner, slot);
@@ -2545,6 +2594,9 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     refinements = fin.cycles;
     inner_total = fin.cycles * inner;
     rel = fin.rel;
+    if (refinements & 1)
+      SMK_CHECK_HIP(hipMemcpyAsync(p, buf[1], (size_t)n * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
   }
   while (!gateable(mg, run, 1)) {
     // defect in FP32 into level-0 rhs32, norm^2 of r into result[0]
