@@ -1443,6 +1443,17 @@ struct SolveState {
   double rel;   // relative residual of the last check
 };
 
+// A slot of the pinned host ring: the state after one check, then the
+// check's sequence number, written last (after a system-scope fence), so
+// the host can poll the slot instead of waiting on an event.
+struct RingSlot {
+  int active;
+  int cycles;
+  double rel;
+  unsigned seq;
+  unsigned pad;
+};
+
 __global__ void k_state_init(SolveState* __restrict__ st) {
   st->active = 1;
   st->cycles = 0;
@@ -1455,16 +1466,24 @@ __global__ void k_state_init(SolveState* __restrict__ st) {
 // `inner` V-cycles per check).
 //
 // The resulting state is also written to *host_out (a slot of a pinned host
-// ring, may be null), so the host learns the outcome without a copy of its
-// own on the stream.
+// ring, may be null) with sequence number seq, so the host learns the
+// outcome without a copy or an event of its own on the stream.
+__device__ __forceinline__ void publish(RingSlot* slot, const SolveState& v, unsigned seq) {
+  slot->active = v.active;
+  slot->cycles = v.cycles;
+  slot->rel = v.rel;
+  __threadfence_system();
+  slot->seq = seq;
+}
+
 __global__ __launch_bounds__(1024) void k_check(const double* __restrict__ partial, int n,
                                                 const double* __restrict__ b2,
                                                 SolveState* __restrict__ st, double tol,
                                                 int cap, int inner,
-                                                SolveState* __restrict__ host_out) {
+                                                RingSlot* __restrict__ host_out, unsigned seq) {
   __shared__ double red[1024 / 64];
   if (st->active == 0) {
-    if (threadIdx.x == 0 && host_out != nullptr) *host_out = *st;
+    if (threadIdx.x == 0 && host_out != nullptr) publish(host_out, *st, seq);
     return;
   }
   double acc = 0.0;
@@ -1478,7 +1497,7 @@ __global__ __launch_bounds__(1024) void k_check(const double* __restrict__ parti
   const bool done = (tol > 0.0 ? rel <= tol : c >= cap) || c * inner >= cap;
   SolveState out{done ? 0 : 1, done ? c : c + 1, rel};
   *st = out;
-  if (host_out != nullptr) *host_out = out;
+  if (host_out != nullptr) publish(host_out, out, seq);
 }
 
 __global__ void k_solid_from_flags(const uint8_t* __restrict__ flags, long n,
@@ -1912,24 +1931,50 @@ inline double optimal_omega(int size) {
   return w < 1.95 ? w : 1.95;
 }
 
-// Queue gated iterations -- iter(k, slot) enqueues iteration k, ending with a
-// k_check on d_state that also writes the state to `slot`, the device
-// address of pinned ring slot k % 4 -- until the device reports the solve
-// done.  The outcome of iteration k-1 is read from its slot while iteration
-// k is already queued, so the GPU never waits for the host; the one
-// iteration queued after the last useful one is gated off and does nothing.
+// Queue gated iterations -- iter(k, slot, seq) enqueues iteration k, ending
+// with a k_check on d_state that also publishes the state to `slot`, the
+// device address of pinned ring slot k % 4, under sequence number `seq` --
+// until the device reports the solve done.  The outcome of iteration k-1 is
+// polled from its slot while iteration k is already queued, so the GPU never
+// waits for the host; the one iteration queued after the last useful one is
+// gated off and does nothing.  Polling replaces an event per iteration (an
+// event record costs the stream a system-scope release, ~5 us of idle GPU).
+//
+// When a slot reports the solve done, that slot holds the final state (the
+// queued no-op iteration leaves it unchanged), so the function returns at
+// once, without waiting for the stream: work the caller queues next follows
+// the no-op iteration on the stream and the GPU does not idle while the host
+// catches up.  While polling, the stream is queried now and then, so an
+// error on the stream ends the wait; a loop that runs out of iterations
+// reads the state after a stream synchronisation.
 template <typename F>
-int run_gated_ring(SolveState* d_state, SolveState* h_ring, SolveState* d_ring, hipEvent_t* ev,
+int run_gated_ring(SolveState* d_state, RingSlot* h_ring, RingSlot* d_ring, unsigned* seq,
                    int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
+  volatile RingSlot* ring = h_ring;
+  const unsigned base = *seq;
+  *seq = base + (unsigned)max_iter + 2u;
   for (int k = 0; k <= max_iter; ++k) {
     const int slot = k & 3;
-    int rc = iter(k, d_ring + slot);
+    int rc = iter(k, d_ring + slot, base + (unsigned)k);
     if (rc) return rc;
-    SMK_CHECK_HIP(hipEventRecord(ev[slot], s));
-    if (k >= 1) {
-      const int prev = (k - 1) & 3;
-      SMK_CHECK_HIP(hipEventSynchronize(ev[prev]));
-      if (reinterpret_cast<volatile SolveState*>(h_ring)[prev].active == 0) break;
+    if (k < 1) continue;
+    const int prev = (k - 1) & 3;
+    const unsigned want = base + (unsigned)(k - 1);
+    for (unsigned spin = 1; ring[prev].seq != want; ++spin) {
+      if ((spin & 1023u) == 0u) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess && ring[prev].seq != want)
+          return fail((int)hipErrorUnknown, "gated solve: check %u never published", want);
+        if (e != hipSuccess && e != hipErrorNotReady)
+          return fail((int)e, "gated solve: %s", hipGetErrorString(e));
+      }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    if (ring[prev].active == 0) {
+      fin->active = ring[prev].active;
+      fin->cycles = ring[prev].cycles;
+      fin->rel = ring[prev].rel;
+      return SMK_OK;
     }
   }
   SMK_CHECK_HIP(hipStreamSynchronize(s));
@@ -1940,17 +1985,17 @@ int run_gated_ring(SolveState* d_state, SolveState* h_ring, SolveState* d_ring, 
 // Pinned read-back ring and events of the single-grid solvers (which take a
 // caller workspace, not a handle): one set per host thread, kept until exit.
 struct GateRing {
-  SolveState* h_ring = nullptr;
-  SolveState* d_ring = nullptr;  // device address of h_ring
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  RingSlot* h_ring = nullptr;
+  RingSlot* d_ring = nullptr;  // device address of h_ring
+  unsigned seq = 1;
 };
 
 static int thread_gate_ring(GateRing** out) {
   thread_local GateRing g;
   if (g.h_ring == nullptr) {
-    SMK_CHECK_HIP(hipHostMalloc((void**)&g.h_ring, 4 * sizeof(SolveState), hipHostMallocDefault));
+    SMK_CHECK_HIP(hipHostMalloc((void**)&g.h_ring, 4 * sizeof(RingSlot), hipHostMallocDefault));
+    memset(g.h_ring, 0, 4 * sizeof(RingSlot));
     SMK_CHECK_HIP(hipHostGetDevicePointer((void**)&g.d_ring, g.h_ring, 0));
-    for (hipEvent_t& e : g.ev) SMK_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   *out = &g;
   return SMK_OK;
@@ -1970,8 +2015,8 @@ int sweeps_gated(SolverWs& w, int iterations, int check_every, double tol, F&& c
   rc = check_launch("k_state_init");
   if (rc) return rc;
   SolveState fin{};
-  rc = run_gated_ring(w.state, g->h_ring, g->d_ring, g->ev, nchunks + 1,
-                      [&](int k, SolveState* slot) -> int {
+  rc = run_gated_ring(w.state, g->h_ring, g->d_ring, &g->seq, nchunks + 1,
+                      [&](int k, RingSlot* slot, unsigned sq) -> int {
     int r = SMK_OK;
     if (k > 0) {
       const int start = (k - 1) * check_every;
@@ -1982,7 +2027,7 @@ int sweeps_gated(SolverWs& w, int iterations, int check_every, double tol, F&& c
     r = chunk(0, &w.state->active);  // the residual partials
     if (r) return r;
     hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, w.partial, nparts, b2, w.state, tol,
-                       nchunks, 1, slot);
+                       nchunks, 1, slot, sq);
     return check_launch("k_check");
   }, &fin, s);
   if (rc) return rc;
@@ -2194,9 +2239,9 @@ struct smk_mg {
   // device-side convergence (gated iterations); each check also writes the
   // state to a pinned ring, one slot per queued iteration
   smk::SolveState* d_state;
-  smk::SolveState* h_ring;
-  smk::SolveState* d_ring;  // device address of h_ring
-  hipEvent_t ev[4];
+  smk::RingSlot* h_ring;
+  smk::RingSlot* d_ring;  // device address of h_ring
+  unsigned ring_seq;
   int small_from[2];  // first level handled by k_vcycle_small: [native dtype, float32]
   int small_mode[2];  // smoother mode small_from was computed for
 };
@@ -2422,7 +2467,8 @@ int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, do
 // iteration queued after the last useful one is gated off and does nothing.
 template <typename F>
 int run_gated(smk_mg* mg, int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
-  return run_gated_ring(mg->d_state, mg->h_ring, mg->d_ring, mg->ev, max_iter, iter, fin, s);
+  return run_gated_ring(mg->d_state, mg->h_ring, mg->d_ring, &mg->ring_seq, max_iter, iter, fin,
+                        s);
 }
 
 // Gated iterations need the finest level's ping-pong flips per V-cycle to be
@@ -2460,7 +2506,7 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
     rc = check_launch("k_state_init");
     if (rc) return rc;
     SolveState fin{};
-    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, SolveState* slot) -> int {
+    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, RingSlot* slot, unsigned sq) -> int {
       int nparts = -1;
       int r = SMK_OK;
       if (k > 0) r = mg_vcycle<T>(mg, 0, false, grun, h, false, mg->partial, &nparts, s);
@@ -2471,7 +2517,8 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
       }
       if (r) return r;
       hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
-                         mg->result + 1, mg->d_state, prm->tolerance, prm->v_cycles, 1, slot);
+                         mg->result + 1, mg->d_state, prm->tolerance, prm->v_cycles, 1, slot,
+                         sq);
       return check_launch("k_check");
     }, &fin, s);
     if (rc) return rc;
@@ -2567,7 +2614,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     // nothing, so the result is in buf[executed refinements & 1]
     double* buf[2] = {p, reinterpret_cast<double*>(L0.p[1])};
     SolveState fin{};
-    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, SolveState* slot) -> int {
+    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, RingSlot* slot, unsigned sq) -> int {
       int r = SMK_OK;
       float* defect = reinterpret_cast<float*>(L0.rhs32);
       if (k > 0) {
@@ -2587,7 +2634,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
       }
       if (r) return r;
       hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
-                         mg->result + 1, mg->d_state, tol, prm->v_cycles, inner, slot);
+                         mg->result + 1, mg->d_state, tol, prm->v_cycles, inner, slot, sq);
       return check_launch("k_check");
     }, &fin, s);
     if (rc) return rc;
@@ -2818,8 +2865,6 @@ int smk_mg_destroy(smk_mg* mg) {
   (void)hipFree(mg->result);
   (void)hipFree(mg->d_state);
   if (mg->h_ring) (void)hipHostFree(mg->h_ring);
-  for (hipEvent_t e : mg->ev)
-    if (e) (void)hipEventDestroy(e);
   delete mg;
   return SMK_OK;
 }
@@ -2869,10 +2914,12 @@ int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W, int num_
   if (!rc) rc = mg_alloc((void**)&mg->result, 4 * sizeof(double));
   if (!rc) rc = mg_alloc((void**)&mg->d_state, sizeof(SolveState));
   if (!rc) {
-    hipError_t e = hipHostMalloc((void**)&mg->h_ring, 4 * sizeof(SolveState), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&mg->d_ring, mg->h_ring, 0);
-    for (int k = 0; k < 4 && e == hipSuccess; ++k)
-      e = hipEventCreateWithFlags(&mg->ev[k], hipEventDisableTiming);
+    hipError_t e = hipHostMalloc((void**)&mg->h_ring, 4 * sizeof(RingSlot), hipHostMallocDefault);
+    if (e == hipSuccess) {
+      memset(mg->h_ring, 0, 4 * sizeof(RingSlot));
+      mg->ring_seq = 1;
+      e = hipHostGetDevicePointer((void**)&mg->d_ring, mg->h_ring, 0);
+    }
     if (e != hipSuccess) rc = fail((int)e, "smk_mg_create: %s", hipGetErrorString(e));
   }
   // level 0 flags from the caller's masks; solid = ~is_fluid (solver.py:38-40)
