@@ -408,8 +408,10 @@ int smk_profile_read(smk_profile_record* out, int n, int reset);
  * (which = 1): 0 = 64x48 LDS tile, 1 = 64x32 LDS tile, 2 = 64x64
  * register-resident region, 3 = 64x48 tile with only p in LDS (rhs and
  * flags in registers), 4 = the same at 64x32, 5 = the same at 64x64 with
- * 512 threads.  which = 2 selects the threads of the single-workgroup
- * small-level V-cycle: 0 = 256, 1 = 512, 2 = 1024.
+ * 512 threads, 6 = 64x16 LDS tile.  which = 3 selects the tile variant of
+ * the coarse levels of at most 512x512 cells (default 6; which = 1 then
+ * covers the larger coarse levels).  which = 2 selects the threads of the
+ * single-workgroup small-level V-cycle: 0 = 256, 1 = 512, 2 = 1024.
  * Results do not depend on the variant. */
 int smk_set_smoother_variant(int dtype, int which, int variant);
 int smk_get_smoother_variant(int dtype, int which);

@@ -228,13 +228,15 @@ def profile_read(reset: bool = True) -> list:
             for k in range(min(n, 64))]
 
 
-SMOOTHER_VARIANTS = {"lds48": 0, "lds32": 1, "reg64": 2, "lds48r": 3, "lds32r": 4, "lds64r": 5}
+SMOOTHER_VARIANTS = {"lds48": 0, "lds32": 1, "reg64": 2, "lds48r": 3, "lds32r": 4, "lds64r": 5,
+                     "lds16": 6}
 SMALL_VARIANTS = {"nt256": 0, "nt512": 1, "nt1024": 2}
-_WHICH = {"fine": 0, "coarse": 1, "small": 2}
+_WHICH = {"fine": 0, "coarse": 1, "small": 2, "coarse_small": 3}
 
 
 def set_smoother_variant(dtype_code: int, which: str, variant: str) -> None:
-    """which: "fine" / "coarse" (tile variants: SMOOTHER_VARIANTS) or
+    """which: "fine" / "coarse" / "coarse_small" (tile variants: SMOOTHER_VARIANTS;
+    "coarse_small" covers coarse levels of at most 512x512 cells) or
     "small" (small-level workgroup size nt256, nt512, nt1024)."""
     table = SMALL_VARIANTS if which == "small" else SMOOTHER_VARIANTS
     call("smk_set_smoother_variant", int(dtype_code), _WHICH[which], table[variant])

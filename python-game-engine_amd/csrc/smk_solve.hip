@@ -1584,13 +1584,29 @@ enum : int {
   VAR_LDS48R = 3,  // 64 x 48 region, p in LDS, rhs / flags in registers, 256 threads
   VAR_LDS32R = 4,  // 64 x 32 region, as VAR_LDS48R
   VAR_LDS64R = 5,  // 64 x 64 region, as VAR_LDS48R with 512 threads
-  NUM_VARIANTS = 6
+  VAR_LDS16 = 6,   // 64 x 16 region, p / rhs / flags in LDS
+  NUM_VARIANTS = 7
 };
-// Defaults per dtype [f32?][coarse?], from timing every variant on the 2048^2
-// V(2,2) workload: float64 finest level 64 x 32 with the rhs in registers
-// (37 us vs 41 us for 64 x 48 in LDS), float32 finest 64 x 64 with the rhs
-// in registers, coarse levels 64 x 32 in LDS.
-static int g_variant[2][2] = {{VAR_LDS32R, VAR_LDS32}, {VAR_LDS64R, VAR_LDS32}};
+// Defaults per dtype [f32?][finest, coarse, small coarse], from timing every
+// variant on the 2048^2 V(2,2) workload: float64 finest level 64 x 32 with
+// the rhs in registers (37 us vs 41 us for 64 x 48 in LDS), float32 finest
+// 64 x 64 with the rhs in registers, coarse levels 64 x 32 in LDS.  Coarse
+// levels of at most SMALL_TILED_CELLS cells occupy a few CUs, so a launch
+// lasts as long as one workgroup: 64 x 16 regions halve the work on that
+// critical path (two cells per lane and pass instead of four).
+static int g_variant[2][3] = {{VAR_LDS32R, VAR_LDS32, VAR_LDS16},
+                              {VAR_LDS64R, VAR_LDS32, VAR_LDS16}};
+constexpr long SMALL_TILED_CELLS = 512L * 512L;
+
+inline int variant_rh(int v);
+
+// The variant of a launch; a region too short for the launch's halo (output
+// rows = region rows - 2R < 2) falls back to the 64 x 32 LDS tile.
+inline int pick_variant(bool f32, bool fine, int H, int W, int R) {
+  const int v = fine ? g_variant[f32][0]
+                     : ((long)H * W <= SMALL_TILED_CELLS ? g_variant[f32][2] : g_variant[f32][1]);
+  return variant_rh(v) - 2 * R < 2 ? VAR_LDS32 : v;
+}
 
 // Threads of the single-workgroup small-level V-cycle, per dtype [f32?].
 static int g_small_nt[2] = {1024, 1024};
@@ -1600,6 +1616,7 @@ inline int variant_rh(int v) {
   switch (v) {
     case VAR_LDS48: case VAR_LDS48R: return 48;
     case VAR_LDS32: case VAR_LDS32R: return 32;
+    case VAR_LDS16: return 16;
     default: return RB_RH;  // VAR_REG64, VAR_LDS64R
   }
 }
@@ -1622,7 +1639,7 @@ int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = fa
     ~Done() { prof_end(ev, tag, bytes, s); }
   } done{ev, tag, ev ? smooth_alg_bytes<T>(mode, a) : 0.0, s};
   if (mode == MODE_RB) {
-    const int v = g_variant[sizeof(T) == 4][fine ? 0 : 1];
+    const int v = pick_variant(sizeof(T) == 4, fine, a.H, a.W, a.R);
     const int rh = variant_rh(v);
     if (v == VAR_REG64 && a.coarse_rhs != nullptr && (a.R & 1))
       return fail(SMK_EINVAL, "restriction needs an even halo (got %d)", a.R);
@@ -1645,6 +1662,8 @@ int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = fa
 #undef SMK_R_LAUNCH
     else if (v == VAR_LDS32)
       hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, 32, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
+    else if (v == VAR_LDS16)
+      hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, 16, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
     else if (fine)
       hipLaunchKernelGGL((k_smooth_fine<T, 48, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
     else
@@ -1659,7 +1678,7 @@ int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = fa
 
 // partial-sum slots one tiled launch can produce
 inline int tiled_blocks(int mode, int H, int W, int R, bool fine = false, bool f32 = false) {
-  const int rh = mode == MODE_RB ? variant_rh(g_variant[f32][fine ? 0 : 1]) : JAC_RH;
+  const int rh = mode == MODE_RB ? variant_rh(pick_variant(f32, fine, H, W, R)) : JAC_RH;
   return (int)(cdiv(W, tile_tx(R)) * cdiv(H, rh - 2 * R));
 }
 
@@ -2327,7 +2346,7 @@ int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_in
     if (a.R < 1) a.R = 1;
     // the register red-black variant restricts (lane pair, row pair) blocks
     if (mode == MODE_RB && last && coarse_rhs != nullptr &&
-        g_variant[sizeof(T) == 4][l == 0 ? 0 : 1] == VAR_REG64)
+        pick_variant(sizeof(T) == 4, l == 0, L.H, L.W, a.R) == VAR_REG64)
       a.R += a.R & 1;
     a.zero_init = (first && zero_init) ? 1 : 0;
     a.gate = gate;
@@ -2730,8 +2749,9 @@ int smk_profile_read(smk_profile_record* out, int n, int reset) {
 
 int smk_set_smoother_variant(int dtype, int which, int variant) {
   SMK_REQUIRE(dtype == SMK_F64 || dtype == SMK_F32, "smk_set_smoother_variant: bad dtype");
-  SMK_REQUIRE(which >= 0 && which <= 2,
-              "smk_set_smoother_variant: which is 0 (fine), 1 (coarse) or 2 (small levels)");
+  SMK_REQUIRE(which >= 0 && which <= 3,
+              "smk_set_smoother_variant: which is 0 (fine), 1 (coarse), 2 (small levels) or "
+              "3 (coarse levels of at most 512x512 cells)");
   if (which == 2) {
     SMK_REQUIRE(variant >= 0 && variant < 3, "smk_set_smoother_variant: bad variant %d", variant);
     g_small_nt[dtype == SMK_F32] = SMALL_NT_CHOICES[variant];
@@ -2739,17 +2759,17 @@ int smk_set_smoother_variant(int dtype, int which, int variant) {
   }
   SMK_REQUIRE(variant >= 0 && variant < NUM_VARIANTS, "smk_set_smoother_variant: bad variant %d",
               variant);
-  g_variant[dtype == SMK_F32][which] = variant;
+  g_variant[dtype == SMK_F32][which == 3 ? 2 : which] = variant;
   return SMK_OK;
 }
 
 int smk_get_smoother_variant(int dtype, int which) {
-  if ((dtype != SMK_F64 && dtype != SMK_F32) || which < 0 || which > 2) return -1;
+  if ((dtype != SMK_F64 && dtype != SMK_F32) || which < 0 || which > 3) return -1;
   if (which == 2) {
     const int nt = g_small_nt[dtype == SMK_F32];
     return nt == 1024 ? 2 : (nt == 512 ? 1 : 0);
   }
-  return g_variant[dtype == SMK_F32][which];
+  return g_variant[dtype == SMK_F32][which == 3 ? 2 : which];
 }
 
 size_t smk_solver_workspace_bytes(int H, int W, int dtype) {

@@ -17,7 +17,7 @@ SMALL_VARIANTS = sorted(_lib.SMALL_VARIANTS)
 H, W = 200, 136
 
 
-def problem():
+def problem(H=H, W=W):
     rng = np.random.default_rng(21)
     solid = np.zeros((H, W), dtype=bool)
     solid[0, :] = solid[-1, :] = True
@@ -38,10 +38,10 @@ MG = dict(v_cycles=3, smoother_type="rb_sor_gauss_seidel", smoother_omega=1.0, n
           smoother_iterations=2, coarsest_iterations=20, coarse_scale="rediscretized")
 
 
-def run_mg(dtype, **extra):
+def run_mg(dtype, H=H, W=W, **extra):
     from cfd_amd.backends.hip import solvers
 
-    d, fmu, fmv = problem()
+    d, fmu, fmv = problem(H, W)
     t = torch.float64 if dtype == "float64" else torch.float32
     out = torch.zeros((H, W), dtype=t, device="cuda")
     dd = torch.from_numpy(d).to(device="cuda", dtype=t)
@@ -95,13 +95,25 @@ def oracle_rbsor():
     return ref
 
 
-@pytest.mark.parametrize("which", ["fine", "coarse"])
+@pytest.mark.parametrize("which", ["fine", "coarse_small"])
 @pytest.mark.parametrize("name", TILE_VARIANTS)
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
 def test_tile_variant_multigrid(dtype, which, name):
     want = oracle_mg() if dtype == "float64" else run_mg(dtype)[0]
     with variant(dtype, which, name):
         got, _ = run_mg(dtype)
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("name", TILE_VARIANTS)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_tile_variant_large_coarse_levels(dtype, name):
+    """Coarse levels above 512x512 cells (a 1040^2 grid: level 1 is 520^2)
+    take the "coarse" variant; the result equals the default variant's."""
+    kw = dict(num_levels=3, v_cycles=2)
+    want, _ = run_mg(dtype, 1040, 1040, **kw)
+    with variant(dtype, "coarse", name):
+        got, _ = run_mg(dtype, 1040, 1040, **kw)
     np.testing.assert_array_equal(got, want)
 
 
@@ -131,7 +143,7 @@ def test_tile_variant_tolerance_solve(name):
     kw = dict(MG, v_cycles=60, tolerance=1e-9)
     ref = np.zeros((H, W))
     n_ref, rel_ref = O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **kw)
-    with variant("float64", "fine", name), variant("float64", "coarse", name):
+    with variant("float64", "fine", name), variant("float64", "coarse_small", name):
         got, info = run_mg("float64", v_cycles=60, tolerance=1e-9)
     assert info["iterations"] == n_ref
     assert info["rel_residual"] == pytest.approx(rel_ref, rel=1e-9)
