@@ -597,6 +597,136 @@ __device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int 
   rtile_epilogue(a, t, bx, by, part, sp, sf, red);
 }
 
+// The p rows of the region go to LDS by LDS-DMA (global_load_lds_dword): no
+// register holds them in flight, so every row of the tile is requested at
+// once and staging costs one memory latency.  One instruction moves 64
+// dwords of one row (wave-uniform LDS destination, lane i -> dword i).
+// Addresses are clamped into the grid, so a slot outside the grid holds the
+// nearest edge value; such slots get zero flags, are never updated, and no
+// in-grid cell reads them (the faces on the grid's edge are closed).
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_dma_p(const SmoothArgs<T>& a, int bx, int by, T* sp) {
+  constexpr int RW = 64, NW = NT / 64;
+  constexpr int EPD = (int)sizeof(T) / 4;  // dwords per value
+  constexpr int CPI = 64 / EPD;            // values per instruction
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int R = a.R, TX = RW - 2 * R, TY = RH - 2 * R;
+  const int gx0 = bx * TX - R, gy0 = by * TY - R;
+#pragma unroll
+  for (int k = 0; k < RH / NW; ++k) {
+    const int ly = wave + k * NW;
+    int gy = gy0 + ly;
+    gy = gy < 0 ? 0 : (gy >= a.H ? a.H - 1 : gy);
+#pragma unroll
+    for (int h = 0; h < EPD; ++h) {
+      int gx = gx0 + h * CPI + lane / EPD;
+      gx = gx < 0 ? 0 : (gx >= a.W ? a.W - 1 : gx);
+      const char* src =
+          reinterpret_cast<const char*>(a.p_in + (size_t)gy * a.W + gx) + 4 * (lane % EPD);
+      __builtin_amdgcn_global_load_lds(
+          src, (__attribute__((address_space(3))) void*)(sp + ly * RW + h * CPI), 4, 0, 0);
+    }
+  }
+}
+
+// rhs and flags of the lane's cells (all pairs at once) into its registers,
+// the flags also into LDS for the restriction.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_stage_rf(const SmoothArgs<T>& a, int bx, int by,
+                                               uint8_t* sf, RTile<T, RH, NT>& t) {
+  using RT = RTile<T, RH, NT>;
+  constexpr int K = RT::K;
+  constexpr uint32_t F_ALL = F_FLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
+  const int R = a.R;
+  const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int px = lane & 1;
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const bool colin = gx >= 0 && gx < a.W;
+  const int gxc = gx < 0 ? 0 : (gx >= a.W ? a.W - 1 : gx);
+  T rv[K][2];
+  uint32_t fv[K][2];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int gy = gy0 + 2 * (wave + k * RT::NW) + q;
+      const int gyc = gy < 0 ? 0 : (gy >= a.H ? a.H - 1 : gy);
+      const size_t g = (size_t)gyc * a.W + gxc;
+      fv[k][q] = a.flags[g];
+      rv[k][q] = a.rhs[g];
+    }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t f[2];
+    T r[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ly = 2 * (wave + k * RT::NW) + q;
+      const int gy = gy0 + ly;
+      const bool in = colin && gy >= 0 && gy < a.H;
+      f[q] = in ? fv[k][q] : 0u;
+      r[q] = in ? rv[k][q] : (T)0;
+      sf[ly * RT::RW + lane] = (uint8_t)f[q];
+    }
+    t.rb[0][k] = px ? r[1] : r[0];
+    t.rb[1][k] = px ? r[0] : r[1];
+    const uint32_t f0 = px ? f[1] : f[0], f1 = px ? f[0] : f[1];
+    t.fw[0][k >> 2] |= f0 << (8 * (k & 3));
+    t.fw[1][k >> 2] |= f1 << (8 * (k & 3));
+    if (__ballot((f0 & F_ALL) == F_ALL) == ~0ull) t.open[0] |= 1u << k;
+    if (__ballot((f1 & F_ALL) == F_ALL) == ~0ull) t.open[1] |= 1u << k;
+  }
+}
+
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void smooth_tile64d_body(const SmoothArgs<T>& a, int bx, int by,
+                                                    int part) {
+  using RT = RTile<T, RH, NT>;
+  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
+  __shared__ T sp[RH * RW];
+  __shared__ uint8_t sf[RH * RW];
+  __shared__ double red[NW];
+  if (a.gate != nullptr && *a.gate == 0) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  RT t;
+  rtile_reset(t);
+  if (a.zero_init) {
+#pragma unroll
+    for (int k = 0; k < RH / NW; ++k) sp[(wave + k * NW) * RW + lane] = (T)0;
+  } else {
+    rtile_dma_p<T, RH, NT>(a, bx, by, sp);
+  }
+  rtile_stage_rf<T, RH, NT>(a, bx, by, sf, t);
+  t.open[0] = __builtin_amdgcn_readfirstlane(t.open[0]);
+  t.open[1] = __builtin_amdgcn_readfirstlane(t.open[1]);
+  __builtin_amdgcn_s_waitcnt(0);  // the LDS-DMA writes have landed
+  __syncthreads();
+  if (a.coarse_e != nullptr) {
+    // prolonged correction on the lane's own fluid cells
+    const int R = a.R;
+    const int TX = RW - 2 * R, TY = RH - 2 * R;
+    const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+    const int px = lane & 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int ly = 2 * (wave + k * NW) + q;
+        // colour of row 2m + q is (q + x) & 1; select the word, not the array
+        const uint32_t fwd = ((q + px) & 1) ? t.fw[1][k >> 2] : t.fw[0][k >> 2];
+        const bool fl = ((fwd >> (8 * (k & 3))) & F_FLUID) != 0u;
+        const T corr = (T)sample_field(a.coarse_e, a.Hc, a.Wc, (gx + 0.5) * 0.5 - 0.5,
+                                       (gy0 + ly + 0.5) * 0.5 - 0.5);
+        const int o = ly * RW + lane;
+        if (fl) sp[o] = sp[o] + corr;
+      }
+    __syncthreads();
+  }
+  rtile_passes(a, t, sp);
+  rtile_epilogue(a, t, bx, by, part, sp, sf, red);
+}
+
 // waves per SIMD the register allocation must allow: 4 when a float64 lane
 // holds six or more row pairs (more would spill), 5 otherwise
 template <typename T, int RH, int NT>
@@ -614,6 +744,20 @@ __global__ __launch_bounds__(NT, (r_waves<T, RH, NT>())) void k_smooth_tile64_r(
   int bx, by;
   xcd_tile(bx, by);
   smooth_tile64r_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
+}
+
+template <typename T, int RH, int NT>
+__global__ __launch_bounds__(NT, 6) void k_smooth_fine_d(const SmoothArgs<T> a) {
+  int bx, by;
+  xcd_tile(bx, by);
+  smooth_tile64d_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
+}
+
+template <typename T, int RH, int NT>
+__global__ __launch_bounds__(NT, 6) void k_smooth_tile64_d(const SmoothArgs<T> a) {
+  int bx, by;
+  xcd_tile(bx, by);
+  smooth_tile64d_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
 }
 
 // Red-black smoothing with the region in registers.  A workgroup of NW
@@ -1585,16 +1729,19 @@ enum : int {
   VAR_LDS32R = 4,  // 64 x 32 region, as VAR_LDS48R
   VAR_LDS64R = 5,  // 64 x 64 region, as VAR_LDS48R with 512 threads
   VAR_LDS16 = 6,   // 64 x 16 region, p / rhs / flags in LDS
-  NUM_VARIANTS = 7
+  VAR_DMA32 = 7,   // 64 x 32 region, p to LDS by LDS-DMA, rhs / flags in registers
+  VAR_DMA64 = 8,   // 64 x 64 region, as VAR_DMA32 with 512 threads
+  NUM_VARIANTS = 9
 };
 // Defaults per dtype [f32?][finest, coarse, small coarse], from timing every
-// variant on the 2048^2 V(2,2) workload: float64 finest level 64 x 32 with
-// the rhs in registers (37 us vs 41 us for 64 x 48 in LDS), float32 finest
-// 64 x 64 with the rhs in registers, coarse levels 64 x 32 in LDS.  Coarse
-// levels of at most SMALL_TILED_CELLS cells occupy a few CUs, so a launch
-// lasts as long as one workgroup: 64 x 16 regions halve the work on that
-// critical path (two cells per lane and pass instead of four).
-static int g_variant[2][3] = {{VAR_LDS32R, VAR_LDS32, VAR_LDS16},
+// variant on the 2048^2 V(2,2) workload: float64 finest level 64 x 64 with p
+// staged by LDS-DMA (35 us; 39 us with p staged through registers, 41 us for
+// 64 x 48 all in LDS), float32 finest 64 x 64 with the rhs in registers,
+// coarse levels 64 x 32 in LDS.  Coarse levels of at most SMALL_TILED_CELLS
+// cells occupy a few CUs, so a launch lasts as long as one workgroup: 64 x 16
+// regions halve the work on that critical path (two cells per lane and pass
+// instead of four).
+static int g_variant[2][3] = {{VAR_DMA64, VAR_LDS32, VAR_LDS16},
                               {VAR_LDS64R, VAR_LDS32, VAR_LDS16}};
 constexpr long SMALL_TILED_CELLS = 512L * 512L;
 
@@ -1615,9 +1762,9 @@ constexpr int SMALL_NT_CHOICES[3] = {256, 512, 1024};
 inline int variant_rh(int v) {
   switch (v) {
     case VAR_LDS48: case VAR_LDS48R: return 48;
-    case VAR_LDS32: case VAR_LDS32R: return 32;
+    case VAR_LDS32: case VAR_LDS32R: case VAR_DMA32: return 32;
     case VAR_LDS16: return 16;
-    default: return RB_RH;  // VAR_REG64, VAR_LDS64R
+    default: return RB_RH;  // VAR_REG64, VAR_LDS64R, VAR_DMA64
   }
 }
 
@@ -1660,6 +1807,18 @@ int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = fa
     else if (v == VAR_LDS64R)
       SMK_R_LAUNCH(64, 512);
 #undef SMK_R_LAUNCH
+#define SMK_D_LAUNCH(RHV, NTV)                                                             \
+  do {                                                                                     \
+    if (fine)                                                                              \
+      hipLaunchKernelGGL((k_smooth_fine_d<T, RHV, NTV>), grid, dim3(NTV), 0, s, a);        \
+    else                                                                                   \
+      hipLaunchKernelGGL((k_smooth_tile64_d<T, RHV, NTV>), grid, dim3(NTV), 0, s, a);      \
+  } while (0)
+    else if (v == VAR_DMA32)
+      SMK_D_LAUNCH(32, 256);
+    else if (v == VAR_DMA64)
+      SMK_D_LAUNCH(64, 512);
+#undef SMK_D_LAUNCH
     else if (v == VAR_LDS32)
       hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, 32, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
     else if (v == VAR_LDS16)
