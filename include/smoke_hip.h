@@ -409,7 +409,8 @@ int smk_profile_read(smk_profile_record* out, int n, int reset);
  * register-resident region, 3 = 64x48 tile with only p in LDS (rhs and
  * flags in registers), 4 = the same at 64x32, 5 = the same at 64x64 with
  * 512 threads, 6 = 64x16 LDS tile, 7 = 64x32 tile with p staged by LDS-DMA
- * (rhs and flags in registers), 8 = the same at 64x64 with 512 threads.
+ * (rhs and flags in registers), 8 = the same at 64x64 with 512 threads,
+ * 9 = the same at 64x16.
  * which = 3 selects the tile variant of
  * the coarse levels of at most 512x512 cells (default 6; which = 1 then
  * covers the larger coarse levels).  which = 2 selects the threads of the

@@ -229,7 +229,7 @@ def profile_read(reset: bool = True) -> list:
 
 
 SMOOTHER_VARIANTS = {"lds48": 0, "lds32": 1, "reg64": 2, "lds48r": 3, "lds32r": 4, "lds64r": 5,
-                     "lds16": 6, "dma32": 7, "dma64": 8}
+                     "lds16": 6, "dma32": 7, "dma64": 8, "dma16": 9}
 SMALL_VARIANTS = {"nt256": 0, "nt512": 1, "nt1024": 2}
 _WHICH = {"fine": 0, "coarse": 1, "small": 2, "coarse_small": 3}
 

@@ -1731,7 +1731,8 @@ enum : int {
   VAR_LDS16 = 6,   // 64 x 16 region, p / rhs / flags in LDS
   VAR_DMA32 = 7,   // 64 x 32 region, p to LDS by LDS-DMA, rhs / flags in registers
   VAR_DMA64 = 8,   // 64 x 64 region, as VAR_DMA32 with 512 threads
-  NUM_VARIANTS = 9
+  VAR_DMA16 = 9,   // 64 x 16 region, as VAR_DMA32
+  NUM_VARIANTS = 10
 };
 // Defaults per dtype [f32?][finest, coarse, small coarse], from timing every
 // variant on the 2048^2 V(2,2) workload: float64 finest level 64 x 64 with p
@@ -1763,7 +1764,7 @@ inline int variant_rh(int v) {
   switch (v) {
     case VAR_LDS48: case VAR_LDS48R: return 48;
     case VAR_LDS32: case VAR_LDS32R: case VAR_DMA32: return 32;
-    case VAR_LDS16: return 16;
+    case VAR_LDS16: case VAR_DMA16: return 16;
     default: return RB_RH;  // VAR_REG64, VAR_LDS64R, VAR_DMA64
   }
 }
@@ -1818,6 +1819,8 @@ int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = fa
       SMK_D_LAUNCH(32, 256);
     else if (v == VAR_DMA64)
       SMK_D_LAUNCH(64, 512);
+    else if (v == VAR_DMA16)
+      SMK_D_LAUNCH(16, 256);
 #undef SMK_D_LAUNCH
     else if (v == VAR_LDS32)
       hipLaunchKernelGGL((k_smooth_tile64<T, MODE_RB, 32, TILE_NT>), grid, dim3(TILE_NT), 0, s, a);
