@@ -203,6 +203,7 @@ int smk_residual_norm2(const void* p, const void* div, const uint8_t* fmu,
 typedef struct smk_mg smk_mg;
 
 #define SMK_SMOOTH_JACOBI 0
+#define SMK_SMOOTH_GS 1 /* lexicographic Gauss-Seidel (a wavefront on the GPU) */
 #define SMK_SMOOTH_RBSOR 2
 
 #define SMK_COARSE_REFERENCE 0    /* restrict(r)/alpha(2h), solver.py:124-125 */

@@ -20,6 +20,7 @@ SMK_F64 = 0
 SMK_F32 = 1
 
 SMK_SMOOTH_JACOBI = 0
+SMK_SMOOTH_GS = 1
 SMK_SMOOTH_RBSOR = 2
 SMK_COARSE_REFERENCE = 0
 SMK_COARSE_REDISCRETIZED = 1

@@ -17,7 +17,8 @@ import ctypes
 import torch
 
 from ._lib import (SMK_COARSE_REDISCRETIZED, SMK_COARSE_REFERENCE, SMK_PREC_MIXED,
-                   SMK_PREC_NATIVE, SMK_SMOOTH_JACOBI, SMK_SMOOTH_RBSOR, MgParams, SolveInfo,
+                   SMK_PREC_NATIVE, SMK_SMOOTH_GS, SMK_SMOOTH_JACOBI, SMK_SMOOTH_RBSOR, MgParams,
+                   SolveInfo,
                    call, lib, stream_of)
 from .operators import _field, _mask, _shape, dtype_code
 
@@ -125,7 +126,8 @@ def residual_norm2(pressure, divergence, face_mask_u, face_mask_v, alpha) -> flo
     return out.value
 
 
-_SMOOTHERS = {"jacobi": SMK_SMOOTH_JACOBI, "rb_sor_gauss_seidel": SMK_SMOOTH_RBSOR}
+_SMOOTHERS = {"jacobi": SMK_SMOOTH_JACOBI, "gauss_seidel": SMK_SMOOTH_GS,
+              "rb_sor_gauss_seidel": SMK_SMOOTH_RBSOR}
 _COARSE = {"reference": SMK_COARSE_REFERENCE, "rediscretized": SMK_COARSE_REDISCRETIZED}
 _PREC = {"native": SMK_PREC_NATIVE, "mixed": SMK_PREC_MIXED}
 
@@ -161,8 +163,10 @@ class MultigridSolver:
 
     Smoothing is fused into the V-cycle kernels, so ``smoother_backend`` is
     accepted for signature compatibility and the smoother always runs on the
-    GPU.  ``smoother_type`` is "jacobi" or "rb_sor_gauss_seidel" (the serial
-    lexicographic Gauss-Seidel smoother has no GPU form).
+    GPU.  ``smoother_type`` is "jacobi", "gauss_seidel" (the lexicographic
+    sweep as an exact wavefront, slow: one workgroup per level) or
+    "rb_sor_gauss_seidel" -- the three solvers the reference's registry
+    holds (solver.py:164 looks the smoother up there).
     """
 
     def __init__(self):
@@ -190,7 +194,7 @@ class MultigridSolver:
         _check(pressure, divergence, face_mask_u, face_mask_v)
         if smoother_type not in _SMOOTHERS:
             raise ValueError(
-                f"multigrid smoother {smoother_type!r} has no GPU form; use one of {list(_SMOOTHERS)}")
+                f"unknown multigrid smoother {smoother_type!r}; use one of {list(_SMOOTHERS)}")
         if coarse_scale not in _COARSE:
             raise ValueError(f"coarse_scale must be one of {list(_COARSE)}")
         if precision not in _PREC:

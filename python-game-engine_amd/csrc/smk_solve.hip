@@ -32,7 +32,7 @@
 
 namespace smk {
 
-enum : int { MODE_RB = 0, MODE_JACOBI = 1 };
+enum : int { MODE_RB = 0, MODE_JACOBI = 1, MODE_GS = 2 };
 
 template <typename T>
 struct SmoothArgs {
@@ -2303,7 +2303,9 @@ template <typename T>
 __global__ __launch_bounds__(GS_NT) void k_gs_wavefront(T* __restrict__ p,
                                                         const T* __restrict__ div,
                                                         const uint8_t* __restrict__ flags, int H,
-                                                        int W, double alpha, int sweeps) {
+                                                        int W, double alpha, int sweeps,
+                                                        const int* __restrict__ gate) {
+  if (gate != nullptr && *gate == 0) return;
   const int tmax = (H - 1) + (W - 1) + 2 * (sweeps - 1);
   for (int t = 0; t <= tmax; ++t) {
     const int over = t - (H - 1) - (W - 1);
@@ -2340,10 +2342,10 @@ int gs_solve(void* p, const void* div, const uint8_t* fmu, const uint8_t* fmv, i
   hipEvent_t ev = prof_begin(s);
   if (dtype == SMK_F64)
     hipLaunchKernelGGL(k_gs_wavefront<double>, dim3(1), dim3(GS_NT), 0, s, (double*)p,
-                       (const double*)div, w.flags, H, W, alpha, iterations);
+                       (const double*)div, w.flags, H, W, alpha, iterations, nullptr);
   else
     hipLaunchKernelGGL(k_gs_wavefront<float>, dim3(1), dim3(GS_NT), 0, s, (float*)p,
-                       (const float*)div, w.flags, H, W, alpha, iterations);
+                       (const float*)div, w.flags, H, W, alpha, iterations, nullptr);
   prof_end(ev, "gauss_seidel", 3.0 * (dtype == SMK_F64 ? 8.0 : 4.0) * H * W * iterations, s);
   return check_launch("k_gs_wavefront");
 }
@@ -2456,6 +2458,7 @@ inline double level_omega(const MgRun& run, const Level& L, bool coarsest = fals
 // First level from which the rest of the hierarchy fits one workgroup's LDS.
 template <typename T>
 int mg_small_from(smk_mg* mg, int mode) {
+  if (mode == MODE_GS) return -1;  // the lexicographic sweep has its own kernel
   const size_t lim = small_lds_limit<T>();
   int Hs[64], Ws[64];
   for (int l = 0; l < mg->nl; ++l) {
@@ -2570,8 +2573,70 @@ int mg_small(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zer
 // result in the last post-smooth launch; *nparts receives the slot count.
 template <typename T>
 int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zero_init,
+              double* norm_partial, int* nparts, hipStream_t s);
+
+template <typename T>
+__global__ void k_fill_gated(T* __restrict__ x, long n, T v, const int* __restrict__ gate) {
+  if (gate != nullptr && *gate == 0) return;
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = v;
+}
+
+// A V-cycle with the lexicographic Gauss-Seidel smoother (solver.py:107,137
+// with smoother_type "gauss_seidel"): the sweeps run in place on the level's
+// current p with the wavefront kernel, the residual-restriction and the
+// prolong-correction are tile launches without smoothing passes (each moves
+// p to the other buffer, two flips per level and cycle).  The level-0
+// residual norm is computed by the caller (*nparts = -1).
+template <typename T>
+int mg_vcycle_gs(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zero_init,
+                 int* nparts, hipStream_t s) {
+  std::vector<Level>& lv = mg->lv;
+  Level& L = lv[l];
+  const bool coarsest = (l == mg->nl - 1);
+  const double alpha = calc_alpha(h, run.rho, run.dt);
+  const long n = (long)L.H * L.W;
+  auto sweep = [&](int iters) -> int {
+    void** pb = use32 ? L.p32 : L.p;
+    T* p = reinterpret_cast<T*>(pb[use32 ? L.cur32 : L.cur]);
+    const T* rhs = reinterpret_cast<const T*>(use32 ? L.rhs32 : L.rhs);
+    if (iters <= 0) return SMK_OK;
+    hipEvent_t ev = prof_begin(s);
+    hipLaunchKernelGGL(k_gs_wavefront<T>, dim3(1), dim3(GS_NT), 0, s, p, rhs, L.flags, L.H, L.W,
+                       alpha, iters, run.gate);
+    prof_end(ev, "gauss_seidel", 3.0 * sizeof(T) * (double)n * iters, s);
+    return check_launch("k_gs_wavefront");
+  };
+  if (nparts != nullptr) *nparts = -1;
+  if (zero_init) {
+    void** pb = use32 ? L.p32 : L.p;
+    hipLaunchKernelGGL(k_fill_gated<T>, dim3(cdiv(n, 256)), dim3(256), 0, s,
+                       reinterpret_cast<T*>(pb[use32 ? L.cur32 : L.cur]), n, (T)0, run.gate);
+    int rc = check_launch("k_fill_gated");
+    if (rc) return rc;
+  }
+  int rc = sweep(coarsest ? run.ci : run.si);
+  if (rc || coarsest) return rc;
+  Level& C = lv[l + 1];
+  T* crhs = reinterpret_cast<T*>(use32 ? C.rhs32 : C.rhs);
+  const double coarse_div = run.rediscretize ? alpha : calc_alpha(h * 2.0, run.rho, run.dt);
+  rc = mg_smooth<T>(lv, l, use32, 0, false, nullptr, 0, 0, crhs, coarse_div, nullptr, nullptr,
+                    alpha, 1.0, MODE_RB, s, run.gate);
+  if (rc) return rc;
+  rc = mg_vcycle<T>(mg, l + 1, use32, run, h * 2.0, true, nullptr, nullptr, s);
+  if (rc) return rc;
+  const T* ce = reinterpret_cast<const T*>(use32 ? C.p32[C.cur32] : C.p[C.cur]);
+  rc = mg_smooth<T>(lv, l, use32, 0, false, ce, C.H, C.W, nullptr, 1.0, nullptr, nullptr, alpha,
+                    1.0, MODE_RB, s, run.gate);
+  if (rc) return rc;
+  return sweep(run.si);
+}
+
+template <typename T>
+int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zero_init,
               double* norm_partial, int* nparts, hipStream_t s) {
   std::vector<Level>& lv = mg->lv;
+  if (run.mode == MODE_GS) return mg_vcycle_gs<T>(mg, l, use32, run, h, zero_init, nparts, s);
   const int from = mg->small_from[use32 ? 1 : 0];
   if (from >= 0 && l >= from) {
     int rc = mg_small<T>(mg, l, use32, run, h, zero_init, s);
@@ -2603,7 +2668,9 @@ int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool ze
 
 inline MgRun make_run(const smk_mg_params* prm, double rho, double dt) {
   MgRun run{};
-  run.mode = prm->smoother == SMK_SMOOTH_JACOBI ? MODE_JACOBI : MODE_RB;
+  run.mode = prm->smoother == SMK_SMOOTH_JACOBI
+                 ? MODE_JACOBI
+                 : (prm->smoother == SMK_SMOOTH_GS ? MODE_GS : MODE_RB);
   run.omega_override = prm->smoother_omega;
   run.coarsest_omega = prm->coarsest_omega;
   run.si = prm->smoother_iterations;
@@ -2658,7 +2725,7 @@ int run_gated(smk_mg* mg, int max_iter, F&& iter, SolveState* fin, hipStream_t s
 // which buffer holds the result where executed ones would.  Coarser levels
 // restart from zero every cycle, so their buffer parity does not matter.
 inline bool gateable(const smk_mg* mg, const MgRun& run, int slot) {
-  if (mg->small_from[slot] == 0) return true;
+  if (mg->small_from[slot] == 0 || run.mode == MODE_GS) return true;
   const int npass = run.mode == MODE_RB ? 2 * run.si : run.si;
   return mg->nl > 1 && npass + 1 <= MAX_R;
 }
@@ -3136,8 +3203,10 @@ int smk_mg_num_levels(const smk_mg* mg) { return mg ? mg->nl : 0; }
 int smk_mg_solve(smk_mg* mg, void* p, const void* div, double rho, double h, double dt,
                  const smk_mg_params* prm, smk_solve_info* info, void* stream) {
   SMK_REQUIRE(mg && p && div && prm, "smk_mg_solve: null pointer");
-  SMK_REQUIRE(prm->smoother == SMK_SMOOTH_JACOBI || prm->smoother == SMK_SMOOTH_RBSOR,
-              "smk_mg_solve: smoother must be SMK_SMOOTH_JACOBI or SMK_SMOOTH_RBSOR");
+  SMK_REQUIRE(prm->smoother == SMK_SMOOTH_JACOBI || prm->smoother == SMK_SMOOTH_RBSOR ||
+                  prm->smoother == SMK_SMOOTH_GS,
+              "smk_mg_solve: smoother must be SMK_SMOOTH_JACOBI, SMK_SMOOTH_GS or "
+              "SMK_SMOOTH_RBSOR");
   SMK_REQUIRE(prm->v_cycles >= 0 && prm->smoother_iterations >= 0 &&
                   prm->coarsest_iterations >= 0,
               "smk_mg_solve: negative iteration count");

@@ -278,6 +278,11 @@ MG_CASES = [
          smoother_iterations=5, coarsest_iterations=7),
     dict(v_cycles=2, smoother_type="jacobi", num_levels=3, smoother_iterations=9,
          coarsest_iterations=11),
+    # lexicographic Gauss-Seidel smoother (the wavefront kernel on every level)
+    dict(v_cycles=2, smoother_type="gauss_seidel", num_levels=4, smoother_iterations=2,
+         coarsest_iterations=12),
+    dict(v_cycles=3, smoother_type="gauss_seidel", num_levels=3, smoother_iterations=3,
+         coarsest_iterations=7, coarse_scale="rediscretized"),
 ]
 
 
@@ -296,6 +301,30 @@ def test_multigrid_bitwise(hip, shape, case, kind):
     mg = sol.MultigridSolver()
     mg(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H, 0.005, **kw)
     np.testing.assert_array_equal(host(out), ref)
+
+
+def test_multigrid_gauss_seidel_tolerance_and_mixed(hip):
+    """Gauss-Seidel-smoothed V-cycles to a tolerance (gated on the device):
+    the oracle's cycle count and result, bit for bit; FP32 V-cycles inside
+    FP64 correction reach the same tolerance."""
+    _, sol = hip
+    H, W = 48, 40
+    _, fmu, fmv, _ = masks_for(H, W, "open")
+    d = rhs_for(H, W, fmu, fmv, 19)
+    kw = dict(v_cycles=60, smoother_type="gauss_seidel", num_levels=3, smoother_iterations=2,
+              coarsest_iterations=20, coarse_scale="rediscretized", tolerance=1e-7)
+    ref = np.zeros((H, W))
+    n_ref, rel_ref = O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **kw)
+    mg = sol.MultigridSolver()
+    out = dev(np.zeros((H, W)))
+    mg(out, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H, 0.005, **kw)
+    info = dict(sol.LAST_SOLVE_INFO)
+    assert info["iterations"] == n_ref and info["converged"] == 1
+    np.testing.assert_array_equal(host(out), ref)
+    mix = dev(np.zeros((H, W)))
+    mg(mix, dev(d), dev(fmu), dev(fmv), 1.0, 1.0 / H, 0.005, precision="mixed", **kw)
+    info = dict(sol.LAST_SOLVE_INFO)
+    assert info["converged"] == 1 and info["rel_residual"] <= 1e-7
 
 
 def test_multigrid_tolerance_and_mixed(hip):
