@@ -60,6 +60,10 @@ def algorithmic_bytes_per_cell(v_cycles_per_step: float, mixed: bool, levels: in
     return 168.0 + 48.0 + v_cycles_per_step * (per_vc64 + 16.0) + 16.0
 
 
+# finest-level launches repeated inside their event pair while profiling
+PROFILE_REPEAT = 8
+
+
 def profile_steps(sim, scene, n: int) -> list:
     """Per kernel family over n extra (untimed) steps: launches, ms, algorithmic
     bytes, recorded by the library with HIP events on the launch stream."""
@@ -69,7 +73,7 @@ def profile_steps(sim, scene, n: int) -> list:
 
     torch.cuda.synchronize()
     _lib.profile_read(reset=True)
-    _lib.profile_enable(True)
+    _lib.profile_enable(True, repeat=PROFILE_REPEAT)
     try:
         for _ in range(n):
             sim.substep(scene.dt)
@@ -100,6 +104,60 @@ def calibrate_copy(nbytes: int = 1 << 30) -> float:
     e1.record()
     torch.cuda.synchronize()
     return 2 * nbytes / (e0.elapsed_time(e1) * 1e6)
+
+
+# kernel-name fragment -> the library's profiling family (the names
+# smk_profile_read reports), for the PMC traffic table
+TRAFFIC_FAMILIES = [
+    ("k_smooth_fine_d<double", "rb_smooth_fine_f64"), ("k_smooth_fine_r<double", "rb_smooth_fine_f64"),
+    ("k_smooth_fine<double", "rb_smooth_fine_f64"),
+    ("k_smooth_fine_d<float", "rb_smooth_fine_f32"), ("k_smooth_fine_r<float", "rb_smooth_fine_f32"),
+    ("k_smooth_fine<float", "rb_smooth_fine_f32"),
+    ("k_smooth_tile64<float, 0", "rb_smooth_coarse_f32"),
+    ("k_smooth_tile64<double, 0", "rb_smooth_coarse_f64"),
+    ("k_vcycle_small<float", "vcycle_small_f32"), ("k_vcycle_small<double", "vcycle_small_f64"),
+    ("k_residual_norm<double", "residual_norm_f64"), ("k_correct_defect<double", "correct_defect_fp64"),
+    ("k_advect_velocity<double", "advect_velocity_f64"), ("k_advect_scalar<double", "advect_scalar_f64"),
+    ("k_pressure_gradient<double", "pressure_gradient_f64"), ("k_divergence<double", "divergence_f64"),
+    ("k_residual_masks<double", "residual_f64"), ("k_apply_forces<double", "apply_forces_f64")]
+
+
+def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30) -> dict:
+    """HBM bytes per launch by kernel family from two rocprofv3 --pmc runs of
+    this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch).  Each counter
+    is scaled by what it reads for the run's own 1 GiB copy (calibrate_copy:
+    copy_bytes read and written), so the table is in true bytes."""
+    import collections
+    import csv
+
+    def family(name):
+        for frag, fam in TRAFFIC_FAMILIES:
+            if frag in name:
+                return fam
+        return None
+
+    per, factor = {}, {}
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = list(csv.DictReader(open(path)))
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for r in rows:
+            f = family(r["Kernel_Name"])
+            if f is not None:
+                agg[f][0] += 1
+                agg[f][1] += float(r["Counter_Value"])
+        per[counter] = {f: (n, v / n) for f, (n, v) in agg.items()}
+        big = [float(r["Counter_Value"]) for r in rows
+               if "copyBuffer" in r["Kernel_Name"] and float(r["Counter_Value"]) * 1024 > copy_bytes / 4]
+        factor[counter] = (copy_bytes / 1024) / (sum(big) / len(big))
+    out = {"calibration": {"copy_bytes_each_way": copy_bytes, "fetch_factor": factor["FETCH_SIZE"],
+                           "write_factor": factor["WRITE_SIZE"]}}
+    for f, (n, fk) in sorted(per["FETCH_SIZE"].items()):
+        wk = per["WRITE_SIZE"].get(f, (0, 0.0))[1]
+        out[f] = {"dispatches": n, "fetch_KiB_per_launch": round(fk, 1),
+                  "write_KiB_per_launch": round(wk, 1),
+                  "hbm_bytes_per_launch": round(fk * 1024 * factor["FETCH_SIZE"]
+                                                + wk * 1024 * factor["WRITE_SIZE"])}
+    return out
 
 
 def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3,
@@ -390,7 +448,19 @@ def main():
                     help="skip the 128^2 Jacobi and 512^2 RB-SOR lines")
     ap.add_argument("--only", choices=("batched", "diff", "small"), default=None,
                     help="run one secondary workload alone and print its JSON (profiling runs)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from the counter_collection.csv files of "
+                         "two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, "
+                         "then exit")
     args = ap.parse_args()
+
+    if args.traffic_from:
+        table = traffic_from_pmc(*args.traffic_from)
+        with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+            json.dump(table, f, indent=1)
+            f.write("\n")
+        print(json.dumps(table, indent=1))
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -392,9 +392,14 @@ int smk_field_stats(const void* field, int rows, int cols, void* out18,
 /* ------------------------------------------------- profiling / tuning */
 /* No reference counterpart: measurement and tuning hooks.
  * smk_profile_enable(1) makes every solver launch record a HIP event pair on
- * its stream, tagged with the launch's algorithmic bytes; smk_profile_read
- * synchronises, aggregates by kernel family (first-seen order) into up to n
- * records, optionally clears, and returns the number of families. */
+ * its stream, tagged with the launch's algorithmic bytes; with on = k > 1 the
+ * finest-level smoothing launches (idempotent) are repeated k times (at most
+ * 64) inside their pair, so the pair's own overhead is spread over k
+ * launches.  While profiling, gated solves wait for each iteration's check
+ * before queueing the next (no gated-off launches in the records).
+ * smk_profile_read synchronises, aggregates by kernel family (first-seen
+ * order) into up to n records, optionally clears, and returns the number of
+ * families. */
 typedef struct smk_profile_record {
   char name[48];
   int launches;

@@ -213,8 +213,10 @@ def stream_of(t: torch.Tensor):
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
-def profile_enable(on: bool) -> None:
-    load().smk_profile_enable(1 if on else 0)
+def profile_enable(on: bool, repeat: int = 1) -> None:
+    """Per-launch HIP-event timing; ``repeat`` > 1 repeats the (idempotent)
+    finest-level smoothing launches inside their event pair."""
+    load().smk_profile_enable((max(1, int(repeat)) if on else 0))
 
 
 def profile_read(reset: bool = True) -> list:

@@ -236,6 +236,26 @@ def _dense_rb(L, p, rhs, alpha, omega, sweeps):
     return p
 
 
+def _dense_gs(L, p, rhs, alpha, sweeps):
+    """Lexicographic Gauss-Seidel as the triangular splitting of
+    test_gauss_seidel_is_triangular_splitting."""
+    N, fl = L["N"], L["fl"]
+    n = N.shape[0]
+    Lo, Up, M = np.tril(N, -1), np.triu(N, 1), np.diag(np.diag(N))
+    A = np.where(fl[:, None], 4 * np.eye(n) - Lo, np.eye(n))
+    b = alpha * rhs
+    for _ in range(sweeps):
+        p = scipy.linalg.solve_triangular(A, np.where(fl, (Up + M) @ p + b, p), lower=True)
+    return p
+
+
+def _dense_jacobi(L, p, rhs, alpha, sweeps):
+    b = alpha * rhs
+    for _ in range(sweeps):
+        p = np.where(L["fl"], (L["N"] @ p + b) / 4.0, p)
+    return p
+
+
 def _restriction_matrix(L):
     H, W = L["H"], L["W"]
     R = np.zeros(((H // 2) * (W // 2), H * W))
@@ -259,12 +279,14 @@ def _prolongation_matrix(L, Hc, Wc):
     return P
 
 
+@pytest.mark.parametrize("smoother", ["rb_sor_gauss_seidel", "gauss_seidel", "jacobi"])
 @pytest.mark.parametrize("kind", ["open", "obstacles"])
 @pytest.mark.parametrize("scale", ["reference", "rediscretized"])
-def test_two_level_vcycle_is_dense_composition(kind, scale):
+def test_two_level_vcycle_is_dense_composition(kind, scale, smoother):
     """One V-cycle on 2 levels = pre-smooth, r = b - A p, coarse rhs =
     R r / alpha_c (reference) or R r / alpha (rediscretised), coarse smoothing
-    from zero, p += P e, post-smooth (solver.py:94-139)."""
+    from zero, p += P e, post-smooth (solver.py:94-139), for each smoother
+    the reference's registry offers (solver.py:164)."""
     H, W = 8, 10
     solid, fmu, fmv, _ = masks(H, W, kind, seed=3)
     L0 = _dense_level(fmu, fmv)
@@ -277,16 +299,18 @@ def test_two_level_vcycle_is_dense_composition(kind, scale):
     h, rho, dt, om = 0.25, 1.3, 0.02, 1.0
     a0, a1 = O.calculate_alpha(h, rho, dt), O.calculate_alpha(2 * h, rho, dt)
     rhs0 = div.ravel()
-    p = _dense_rb(L0, p0.ravel(), rhs0, a0, om, 2)
+    smooth = {"rb_sor_gauss_seidel": lambda L, p, r, a, k: _dense_rb(L, p, r, a, om, k),
+              "gauss_seidel": _dense_gs, "jacobi": _dense_jacobi}[smoother]
+    p = smooth(L0, p0.ravel(), rhs0, a0, 2)
     A0 = 4 * np.eye(H * W) - L0["N"]
     r = np.where(L0["fl"], a0 * rhs0 - A0 @ p, 0.0)
     rc = _restriction_matrix(L0) @ r / (a1 if scale == "reference" else a0)
-    e = _dense_rb(L1, np.zeros(rc.size), rc, a1, om, 50)
+    e = smooth(L1, np.zeros(rc.size), rc, a1, 50)
     p = p + _prolongation_matrix(L0, H // 2, W // 2) @ e
-    p = _dense_rb(L0, p, rhs0, a0, om, 2)
+    p = smooth(L0, p, rhs0, a0, 2)
     out = p0.copy()
     O.multigrid_solver(out, div, fmu, fmv, rho, h, dt, v_cycles=1,
-                       smoother_type="rb_sor_gauss_seidel", num_levels=2, smoother_iterations=2,
+                       smoother_type=smoother, num_levels=2, smoother_iterations=2,
                        coarsest_iterations=50, smoother_omega=om, coarse_scale=scale)
     close(out.ravel(), p, 1e-11)
 
