@@ -2479,7 +2479,14 @@ inline double level_omega(const MgRun& run, const Level& L, bool coarsest = fals
   return run.omega_override > 0.0 ? run.omega_override : optimal_omega(n);
 }
 
-// First level from which the rest of the hierarchy fits one workgroup's LDS.
+// First level from which the small-level kernel takes the rest of the
+// hierarchy: the first that fits one workgroup's LDS.  When tiled levels
+// precede it anyway, the kernel starts no higher than SMALL_TOP_CELLS cells:
+// it runs on one CU, where a 64x64 level costs more (12.8 us of smoothing,
+// restriction and prolongation) than two tiled launches (~10 us); measured
+// 1 % per 2048^2 step.  A hierarchy that fits entirely stays one launch.
+constexpr long SMALL_TOP_CELLS = 32L * 32L;
+
 template <typename T>
 int mg_small_from(smk_mg* mg, int mode) {
   if (mode == MODE_GS) return -1;  // the lexicographic sweep has its own kernel
@@ -2489,11 +2496,15 @@ int mg_small_from(smk_mg* mg, int mode) {
     Hs[l] = mg->lv[l].H;
     Ws[l] = mg->lv[l].W;
   }
-  for (int from = 0; from < mg->nl; ++from) {
+  int from0 = -1;
+  for (int from = 0; from < mg->nl && from0 < 0; ++from) {
     if (mg->nl - from > SMALL_MAX) continue;
-    if (small_lds_bytes(Hs + from, Ws + from, mg->nl - from, sizeof(T), mode) <= lim) return from;
+    if (small_lds_bytes(Hs + from, Ws + from, mg->nl - from, sizeof(T), mode) <= lim) from0 = from;
   }
-  return -1;  // the coarsest level alone does not fit: tiled kernels only
+  if (from0 <= 0) return from0;  // -1: the coarsest level alone does not fit
+  for (int from = from0; from < mg->nl; ++from)
+    if ((long)Hs[from] * Ws[from] <= SMALL_TOP_CELLS) return from;
+  return from0;
 }
 
 // Smooth level l with npass passes (tiled kernels), optionally prolonging a
