@@ -420,7 +420,11 @@ int smk_profile_read(smk_profile_record* out, int n, int reset);
  * which = 3 selects the tile variant of
  * the coarse levels of at most 512x512 cells (default 6; which = 1 then
  * covers the larger coarse levels).  which = 2 selects the threads of the
- * single-workgroup small-level V-cycle: 0 = 256, 1 = 512, 2 = 1024.
+ * single-workgroup small-level V-cycle: 0 = 256, 1 = 512, 2 = 1024; 3 = 256
+ * and 4 = 512 also run its red-black levels of at most 32x32 cells, the
+ * coarsest included, on one wavefront with the cells in registers; 5 = 256,
+ * 6 = 512 and 7 = 1024 run only a coarsest level of at most 16x16 cells
+ * that way.
  * Results do not depend on the variant. */
 int smk_set_smoother_variant(int dtype, int which, int variant);
 int smk_get_smoother_variant(int dtype, int which);

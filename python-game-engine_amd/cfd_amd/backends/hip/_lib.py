@@ -233,13 +233,19 @@ def profile_read(reset: bool = True) -> list:
 
 SMOOTHER_VARIANTS = {"lds48": 0, "lds32": 1, "reg64": 2, "lds48r": 3, "lds32r": 4, "lds64r": 5,
                      "lds16": 6, "dma32": 7, "dma64": 8, "dma16": 9}
-SMALL_VARIANTS = {"nt256": 0, "nt512": 1, "nt1024": 2}
+# small-level kernel: workgroup size; "w": red-black levels of at most 32x32
+# cells on one wavefront (registers, DPP / permlane exchange); "c": only a
+# coarsest level of at most 16x16 cells that way
+SMALL_VARIANTS = {"nt256": 0, "nt512": 1, "nt1024": 2, "nt256w": 3, "nt512w": 4, "nt256c": 5,
+                  "nt512c": 6, "nt1024c": 7}
 _WHICH = {"fine": 0, "coarse": 1, "small": 2, "coarse_small": 3}
 
 
 def set_smoother_variant(dtype_code: int, which: str, variant: str) -> None:
     """which: "fine" / "coarse" / "coarse_small" (tile variants: SMOOTHER_VARIANTS;
     "coarse_small" covers coarse levels of at most 512x512 cells) or
-    "small" (small-level workgroup size nt256, nt512, nt1024)."""
+    "small" (small-level workgroup size nt256, nt512, nt1024; nt256w and nt512w also
+    smooth levels of at most 32x32 cells on one wavefront, the "c" forms only a
+    coarsest level of at most 16x16 cells)."""
     table = SMALL_VARIANTS if which == "small" else SMOOTHER_VARIANTS
     call("smk_set_smoother_variant", int(dtype_code), _WHICH[which], table[variant])

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(SMALL_NT) void k_b_vcycle_small(
     const SmallArgs<float>* __restrict__ args, const int* __restrict__ scene_of,
     const int* __restrict__ active) {
   if (!active[scene_of[blockIdx.x]]) return;
-  vcycle_small_body<float, MODE_RB, SMALL_NT>(args[blockIdx.x]);
+  vcycle_small_body<float, MODE_RB, SMALL_NT, 0>(args[blockIdx.x]);
 }
 
 __global__ void k_b_init(int n, int* __restrict__ active, int* __restrict__ cycles) {

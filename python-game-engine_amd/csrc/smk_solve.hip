@@ -355,8 +355,8 @@ __device__ __forceinline__ void rtile_issue(const SmoothArgs<T>& a, int bx, int 
     }
 }
 
-// Apply the prolonged correction, zero what lies outside the grid, and place
-// p and the flags in LDS and the rhs and flags in the lane's registers.
+// Zero what lies outside the grid, and place p and the flags in LDS and the
+// rhs and flags in the lane's registers.
 template <typename T, int RH, int NT, int KH>
 __device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int by, int h0,
                                              T (&pv)[KH][2], const T (&rv)[KH][2],
@@ -370,17 +370,6 @@ __device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int
   const int px = lane & 1;
   const int gx = bx * TX - R + lane, gy0 = by * TY - R;
   const bool colin = gx >= 0 && gx < a.W;
-  if (a.coarse_e != nullptr) {
-#pragma unroll
-    for (int k = 0; k < KH; ++k)
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
-        const T corr = (T)sample_field(a.coarse_e, a.Hc, a.Wc, (gx + 0.5) * 0.5 - 0.5,
-                                       (gy + 0.5) * 0.5 - 0.5);
-        pv[k][q] = pv[k][q] + ((fv[k][q] & F_FLUID) ? corr : (T)0);
-      }
-  }
 #pragma unroll
   for (int k = 0; k < KH; ++k) {
     uint32_t f[2];
@@ -442,32 +431,71 @@ __device__ __forceinline__ T rtile_nsum(const RTile<T, RH, NT>& t, int c, int k,
 // keeps every register array index constant).  Cells of one colour never
 // neighbour each other: every new value of the lane is computed before any
 // is stored.
-template <int C, typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_sweep(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                            T* sp, int lo) {
+// OPEN: every pair of the lane's colour-C cells is open in every lane (no
+// selects), decided once per pass rather than per cell, so the pass is one
+// straight run of code.  The LDS reads of a batch of KB cells are issued
+// before any is used; KB = 1 measured fastest (larger batches raise the
+// registers past the occupancy of three 512-thread workgroups per CU).  A
+// batch's new values are stored before the next batch reads: a cell's reads
+// are its own value and cells of the other colour, which the pass does not
+// write.
+template <int C, bool OPEN, typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                              T* sp, int lo) {
   using RT = RTile<T, RH, NT>;
   constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
+  constexpr int KB = 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int par = (lane + C) & 1;
   const bool xin = lane >= lo && lane < RW - lo;
-  T nv[K];
-  int oo[K];
-  bool st[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const int ly = 2 * (wave + k * NW) + par;
-    const bool valid = xin && ly >= lo && ly < RH - lo;
-    const int o = valid ? ly * RW + lane : RW + 1;
-    T ce;
-    bool fl;
-    const T ns = rtile_nsum(t, C, k, sp, o, ce, fl);
-    nv[k] = (T)relax_value(0, ce, ns, t.rb[C][k], a.alpha, a.omega);
-    oo[k] = o;
-    st[k] = valid && fl;
+  for (int k0 = 0; k0 < K; k0 += KB) {
+    T ce[KB], lv[KB], rv[KB], dv[KB], uv[KB];
+    int oo[KB];
+    bool valid[KB];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const int ly = 2 * (wave + (k0 + j) * NW) + par;
+      valid[j] = xin && ly >= lo && ly < RH - lo;
+      const int o = valid[j] ? ly * RW + lane : RW + 1;
+      oo[j] = o;
+      ce[j] = sp[o];
+      lv[j] = sp[o - 1];
+      rv[j] = sp[o + 1];
+      dv[j] = sp[o - RW];
+      uv[j] = sp[o + RW];
+    }
+#pragma unroll
+    for (int j = 0; j < KB; ++j) {
+      const int k = k0 + j;
+      T ns;
+      bool fl;
+      if (OPEN) {
+        ns = lv[j] + rv[j] + dv[j] + uv[j];
+        fl = true;
+      } else {
+        const uint32_t* fwc = t.fw[C];
+        const T l = fbit(fwc, k, F_LEFT) ? lv[j] : ce[j];
+        const T r = fbit(fwc, k, F_RIGHT) ? rv[j] : ce[j];
+        const T d = fbit(fwc, k, F_DOWN) ? dv[j] : ce[j];
+        const T u = fbit(fwc, k, F_UP) ? uv[j] : ce[j];
+        ns = l + r + d + u;
+        fl = fbit(fwc, k, F_FLUID);
+      }
+      const T nv = (T)relax_value(0, ce[j], ns, t.rb[C][k], a.alpha, a.omega);
+      if (valid[j] && fl) sp[oo[j]] = nv;
+    }
   }
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-    if (st[k]) sp[oo[k]] = nv[k];
+}
+
+template <int C, typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_sweep(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                            T* sp, int lo) {
+  constexpr uint32_t ALL = (1u << RTile<T, RH, NT>::K) - 1u;
+  if ((t.open[C] & ALL) == ALL)  // wave-uniform
+    rtile_sweep_k<C, true>(a, t, sp, lo);
+  else
+    rtile_sweep_k<C, false>(a, t, sp, lo);
 }
 
 // npass colour passes; the valid window shrinks by one cell per pass.
@@ -569,16 +597,106 @@ __device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTi
   }
 }
 
+// The coarse-grid patch the bilinear prolongation of a 64 x RH region reads:
+// region column gx samples the coarse grid at x = (gx + 0.5) / 2 - 0.5, so the
+// region's samples touch coarse columns (gx0 - 1) >> 1 .. that + 33 (and rows
+// likewise), clamped into the coarse grid exactly as sample_field clamps.
+// Staged in LDS with the tile, the prolongation then costs no second trip to
+// memory.
+template <typename T, int RH, int NT>
+struct CPatch {
+  static constexpr int PW = 34, PH = RH / 2 + 2, N = PH * PW;
+  static constexpr int PER = (N + NT - 1) / NT;  // values per thread
+};
+
+// Issue the patch loads (coarse columns cx0 .., rows cy0 ..) into registers.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void cpatch_issue(const SmoothArgs<T>& a, int bx, int by,
+                                             T (&v)[CPatch<T, RH, NT>::PER]) {
+  using CP = CPatch<T, RH, NT>;
+  const int R = a.R, TX = 64 - 2 * R, TY = RH - 2 * R;
+  const int cx0 = (bx * TX - R - 1) >> 1, cy0 = (by * TY - R - 1) >> 1;
+#pragma unroll
+  for (int i = 0; i < CP::PER; ++i) {
+    const int idx = threadIdx.x + i * NT;
+    const int py = idx / CP::PW, px = idx - py * CP::PW;
+    int gy = cy0 + py, gx = cx0 + px;
+    gy = gy < 0 ? 0 : (gy >= a.Hc ? a.Hc - 1 : gy);
+    gx = gx < 0 ? 0 : (gx >= a.Wc ? a.Wc - 1 : gx);
+    v[i] = idx < CP::N ? a.coarse_e[(size_t)gy * a.Wc + gx] : (T)0;
+  }
+}
+
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void cpatch_store(const T (&v)[CPatch<T, RH, NT>::PER], T* spatch) {
+  using CP = CPatch<T, RH, NT>;
+#pragma unroll
+  for (int i = 0; i < CP::PER; ++i) {
+    const int idx = threadIdx.x + i * NT;
+    if (idx < CP::N) spatch[idx] = v[i];
+  }
+}
+
+// The prolonged correction added to the lane's fluid cells in LDS (the p of
+// the region is staged; the patch is in LDS; a barrier follows).  The
+// sample position of fine index g is g / 2 - 0.25, never an integer, so
+// sample_field's floor and ceil are (g - 1) >> 1 and one more; with the same
+// clamping, weights (x minus the clamped floor, exact) and lerps the values
+// are sample_field's.  The column part is the lane's, the row part the pair's.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_prolong(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                              int bx, int by, T* sp, const T* spatch) {
+  using RT = RTile<T, RH, NT>;
+  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW, PW = CPatch<T, RH, NT>::PW;
+  const int R = a.R;
+  const int TX = RW - 2 * R, TY = RH - 2 * R;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const int cx0 = (bx * TX - R - 1) >> 1, cy0 = (gy0 - 1) >> 1;
+  const int px = lane & 1;
+  const int xf = (gx - 1) >> 1;
+  const int x0 = min(max(xf, 0), a.Wc - 1), x1 = min(max(xf + 1, 0), a.Wc - 1);
+  const double tx = ((gx + 0.5) * 0.5 - 0.5) - (double)x0;
+  const T* col0 = spatch + (x0 - cx0);
+  const T* col1 = spatch + (x1 - cx0);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ly = 2 * (wave + k * NW) + q;
+      const int gy = gy0 + ly;
+      const int yf = (gy - 1) >> 1;
+      const int y0 = min(max(yf, 0), a.Hc - 1), y1 = min(max(yf + 1, 0), a.Hc - 1);
+      const double ty = ((gy + 0.5) * 0.5 - 0.5) - (double)y0;
+      const int r0 = (y0 - cy0) * PW, r1 = (y1 - cy0) * PW;
+      const double c00 = (double)col0[r0], c10 = (double)col1[r0];
+      const double c01 = (double)col0[r1], c11 = (double)col1[r1];
+      const double c0 = c00 + (c10 - c00) * tx;
+      const double c1 = c01 + (c11 - c01) * tx;
+      const T corr = (T)(c0 + (c1 - c0) * ty);
+      // colour of row 2m + q is (q + x) & 1; select the word, not the array
+      const uint32_t fwd = ((q + px) & 1) ? t.fw[1][k >> 2] : t.fw[0][k >> 2];
+      const bool fl = ((fwd >> (8 * (k & 3))) & F_FLUID) != 0u;
+      const int o = ly * RW + lane;
+      if (fl) sp[o] = sp[o] + corr;
+    }
+}
+
 template <typename T, int RH, int NT>
 __device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int bx, int by,
                                                     int part) {
   using RT = RTile<T, RH, NT>;
+  using CP = CPatch<T, RH, NT>;
   __shared__ T sp[RH * RT::RW];
   __shared__ uint8_t sf[RH * RT::RW];
   __shared__ double red[RT::NW];
+  __shared__ T spatch[CP::N];
   if (a.gate != nullptr && *a.gate == 0) return;
   RT t;
   rtile_reset(t);
+  const bool prolong = a.coarse_e != nullptr;
+  T cv[CP::PER];
+  if (prolong) cpatch_issue<T, RH, NT>(a, bx, by, cv);  // overlaps the tile's loads
   // in two halves of the lane's pairs, to bound the registers of loads in
   // flight
   constexpr int K = RT::K;
@@ -588,11 +706,16 @@ __device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int 
     T pv[KH][2], rv[KH][2];
     uint32_t fv[KH][2];
     rtile_issue<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv);
+    if (h0 == 0 && prolong) cpatch_store<T, RH, NT>(cv, spatch);
     rtile_commit<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv, sp, sf, t);
   }
   t.open[0] = __builtin_amdgcn_readfirstlane(t.open[0]);
   t.open[1] = __builtin_amdgcn_readfirstlane(t.open[1]);
   __syncthreads();
+  if (prolong) {
+    rtile_prolong<T, RH, NT>(a, t, bx, by, sp, spatch);
+    __syncthreads();
+  }
   rtile_passes(a, t, sp);
   rtile_epilogue(a, t, bx, by, part, sp, sf, red);
 }
@@ -683,10 +806,12 @@ template <typename T, int RH, int NT>
 __device__ __forceinline__ void smooth_tile64d_body(const SmoothArgs<T>& a, int bx, int by,
                                                     int part) {
   using RT = RTile<T, RH, NT>;
-  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
+  using CP = CPatch<T, RH, NT>;
+  constexpr int RW = RT::RW, NW = RT::NW;
   __shared__ T sp[RH * RW];
   __shared__ uint8_t sf[RH * RW];
   __shared__ double red[NW];
+  __shared__ T spatch[CP::N];
   if (a.gate != nullptr && *a.gate == 0) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   RT t;
@@ -697,30 +822,17 @@ __device__ __forceinline__ void smooth_tile64d_body(const SmoothArgs<T>& a, int 
   } else {
     rtile_dma_p<T, RH, NT>(a, bx, by, sp);
   }
+  const bool prolong = a.coarse_e != nullptr;
+  T cv[CP::PER];
+  if (prolong) cpatch_issue<T, RH, NT>(a, bx, by, cv);
   rtile_stage_rf<T, RH, NT>(a, bx, by, sf, t);
+  if (prolong) cpatch_store<T, RH, NT>(cv, spatch);
   t.open[0] = __builtin_amdgcn_readfirstlane(t.open[0]);
   t.open[1] = __builtin_amdgcn_readfirstlane(t.open[1]);
   __builtin_amdgcn_s_waitcnt(0);  // the LDS-DMA writes have landed
   __syncthreads();
-  if (a.coarse_e != nullptr) {
-    // prolonged correction on the lane's own fluid cells
-    const int R = a.R;
-    const int TX = RW - 2 * R, TY = RH - 2 * R;
-    const int gx = bx * TX - R + lane, gy0 = by * TY - R;
-    const int px = lane & 1;
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int ly = 2 * (wave + k * NW) + q;
-        // colour of row 2m + q is (q + x) & 1; select the word, not the array
-        const uint32_t fwd = ((q + px) & 1) ? t.fw[1][k >> 2] : t.fw[0][k >> 2];
-        const bool fl = ((fwd >> (8 * (k & 3))) & F_FLUID) != 0u;
-        const T corr = (T)sample_field(a.coarse_e, a.Hc, a.Wc, (gx + 0.5) * 0.5 - 0.5,
-                                       (gy0 + ly + 0.5) * 0.5 - 0.5);
-        const int o = ly * RW + lane;
-        if (fl) sp[o] = sp[o] + corr;
-      }
+  if (prolong) {
+    rtile_prolong<T, RH, NT>(a, t, bx, by, sp, spatch);
     __syncthreads();
   }
   rtile_passes(a, t, sp);
@@ -765,16 +877,78 @@ __global__ __launch_bounds__(NT, 6) void k_smooth_tile64_d(const SmoothArgs<T> a
 // w owns rows [w*BR, (w+1)*BR).  Rows are held in pairs (2m, 2m+1) as
 // A[m] = the colour-0 cell and B[m] = the colour-1 cell of the pair, so a
 // colour pass updates exactly one register per pair and lane:
-//   horizontal neighbours: the other colour of lanes x-1 / x+1 (__shfl)
+//   horizontal neighbours: the other colour of lanes x-1 / x+1 (DPP)
 //   vertical neighbours:   the other colour in this lane's own pairs
 //                          (pair m-1 / m+1), across wave bands via a 4 KB
 //                          LDS exchange of the band-edge values.
 // LDS holds only those edges, so occupancy is set by registers.  With R even
 // the 2x2 restriction blocks coincide with (lane pair, row pair).
+
+// Apply a 32-bit lane operation to every dword of a value.
+template <typename T, typename F>
+__device__ __forceinline__ T per_dword(T v, F f) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized values only");
+  uint32_t w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = f(w[i]);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+// Lane l receives lane l-1's value, lane 0 keeps its own (DPP wave_shr:1;
+// the result of __shfl_up(v, 1, 64) without a trip through the LDS
+// crossbar).  All 64 lanes must be active.
 template <typename T>
-__device__ __forceinline__ T from_left(T v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ T from_left(T v) {
+  return per_dword(v, [](uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x138, 0xf, 0xf, false);
+  });
+}
+// Lane l receives lane l+1's value, lane 63 keeps its own (DPP wave_shl:1).
 template <typename T>
-__device__ __forceinline__ T from_right(T v) { return __shfl_down(v, 1, 64); }
+__device__ __forceinline__ T from_right(T v) {
+  return per_dword(v, [](uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x130, 0xf, 0xf, false);
+  });
+}
+
+// Strip exchange for a wavefront split into 64 / COLS strips of COLS lanes:
+// lane l receives v from lane l - COLS if `down`, else from lane l + COLS
+// (lanes whose source lies outside the wave receive an unspecified value).
+// gfx950's v_permlane16_swap / v_permlane32_swap: for (old, src),
+// permlane16 returns rows [old0, src0, old2, src2] and [old1, src1, old3,
+// src3] (rows of 16 lanes), permlane32 returns [old_lo, src_lo] and
+// [old_hi, src_hi] (halves of 32 lanes).
+template <int COLS>
+__device__ __forceinline__ uint32_t strip_exchange_u32(uint32_t v, bool down) {
+  const int lane = threadIdx.x & 63;
+  if constexpr (COLS == 32) {
+    const auto h = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return lane >= 32 ? h[0] : h[1];  // the other half holds the other strip
+  } else {
+    static_assert(COLS == 16, "strips of 16 or 32 lanes");
+    // a = rows [v0 v0 v2 v2], b = [v1 v1 v3 v3]
+    const auto ab = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    // c = [v0 v0 v1 v1], d = [v2 v2 v3 v3]
+    const auto cd = __builtin_amdgcn_permlane32_swap(ab[0], ab[1], false, false);
+    const int row = lane >> 4;
+    // from below: row 1 <- v0 (a), row 2 <- v1 (c), row 3 <- v2 (a)
+    // from above: row 0 <- v1 (b), row 1 <- v2 (d), row 2 <- v3 (b)
+    return down ? (row == 2 ? cd[0] : ab[0]) : (row == 1 ? cd[1] : ab[1]);
+  }
+}
+template <int COLS, typename T>
+__device__ __forceinline__ T strip_exchange(T v, bool down) {
+  uint32_t w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = strip_exchange_u32<COLS>(w[i], down);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
 
 template <typename T>
 __device__ __forceinline__ T rb_relax(T ce, T l, T r, T d, T u, T b, uint32_t f, double alpha,
@@ -1193,9 +1367,110 @@ __device__ __forceinline__ void small_smooth_rb_reg(T* p, const T* rhs, const ui
   __syncthreads();
 }
 
-template <typename T, int MODE, int NT>
+constexpr int WAVE_COARSEST = 1, WAVE_LEVELS = 2;
+
+// Red-black passes over a level of at most COLS x (128 / COLS * P) cells on
+// one wavefront with the cells in registers.  Lane = x + COLS * s owns
+// column x of strip s, rows [2Ps, 2P(s+1)), held as pairs m (rows 2Ps + 2m,
+// 2Ps + 2m + 1): A[m] / B[m] is the pair's colour-0 / colour-1 cell, at row
+// 2Ps + 2m + px / 2Ps + 2m + 1 - px with px = x & 1.  A colour pass updates
+// one register per pair.  Horizontal neighbours are the other colour of
+// lanes l -/+ 1 (DPP); vertical ones the other colour in the lane's own
+// pairs and, across a strip edge, of the lane COLS below or above
+// (strip_exchange), so a pass touches neither LDS nor a barrier.  Neighbour
+// values from outside the level are never used: the flags of the level's
+// border cells have those neighbour bits cleared.  The per-cell arithmetic is
+// rb_relax's.  All 64 lanes of the wave must call it.
+template <typename T, int COLS, int P>
+__device__ __forceinline__ void wave_smooth_rb(T* p, const T* rhs, const uint8_t* fl, int H, int W,
+                                               double alpha, double omega, int npass) {
+  const int lane = threadIdx.x & 63;
+  const int x = lane % COLS, s = lane / COLS;
+  const int px = x & 1;
+  const int r0 = 2 * P * s;
+  const bool lin = x < W;
+  T A[P], B[P], RA[P], RB[P];
+  uint32_t FA[P], FB[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int ya = r0 + 2 * m + px, yb = r0 + 2 * m + 1 - px;
+    const bool oka = lin && ya < H, okb = lin && yb < H;
+    const int oa = oka ? ya * W + x : 0, ob = okb ? yb * W + x : 0;
+    A[m] = oka ? p[oa] : (T)0;
+    B[m] = okb ? p[ob] : (T)0;
+    RA[m] = oka ? rhs[oa] : (T)0;
+    RB[m] = okb ? rhs[ob] : (T)0;
+    FA[m] = oka ? fl[oa] : 0u;
+    FB[m] = okb ? fl[ob] : 0u;
+  }
+  for (int pass = 0; pass < npass; ++pass) {
+    if ((pass & 1) == 0) {
+      // px = 0: A[0]'s lower neighbour is B[P-1] of the strip below;
+      // px = 1: A[P-1]'s upper neighbour is B[0] of the strip above
+      const T e = strip_exchange<COLS>(px ? B[0] : B[P - 1], px == 0);
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const T d = px ? B[m] : (m > 0 ? B[m > 0 ? m - 1 : 0] : e);
+        const T u = px ? (m + 1 < P ? B[m + 1 < P ? m + 1 : m] : e) : B[m];
+        const T nv = rb_relax(A[m], from_left(B[m]), from_right(B[m]), d, u, RA[m], FA[m], alpha,
+                              omega);
+        A[m] = (FA[m] & F_FLUID) ? nv : A[m];
+      }
+    } else {
+      // px = 1: B[0]'s lower neighbour is A[P-1] of the strip below;
+      // px = 0: B[P-1]'s upper neighbour is A[0] of the strip above
+      const T e = strip_exchange<COLS>(px ? A[P - 1] : A[0], px == 1);
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const T d = px ? (m > 0 ? A[m > 0 ? m - 1 : 0] : e) : A[m];
+        const T u = px ? A[m] : (m + 1 < P ? A[m + 1 < P ? m + 1 : m] : e);
+        const T nv = rb_relax(B[m], from_left(A[m]), from_right(A[m]), d, u, RB[m], FB[m], alpha,
+                              omega);
+        B[m] = (FB[m] & F_FLUID) ? nv : B[m];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int ya = r0 + 2 * m + px, yb = r0 + 2 * m + 1 - px;
+    if (lin && ya < H) p[ya * W + x] = A[m];
+    if (lin && yb < H) p[yb * W + x] = B[m];
+  }
+}
+
+// Red-black passes over a level of at most 32 x 32 cells on the calling
+// wavefront (all 64 lanes): strips of 16 lanes for levels up to 16 x 16,
+// of 32 lanes up to 32 x 32, with the pairs per lane as a compile-time count.
+template <typename T>
+__device__ __forceinline__ void wave_smooth_level(T* p, const T* rhs, const uint8_t* fl, int H,
+                                                  int W, double alpha, double omega, int npass) {
+  if (W <= 16 && H <= 16) {
+    if (H <= 8)
+      wave_smooth_rb<T, 16, 1>(p, rhs, fl, H, W, alpha, omega, npass);
+    else
+      wave_smooth_rb<T, 16, 2>(p, rhs, fl, H, W, alpha, omega, npass);
+    return;
+  }
+  switch ((H + 3) / 4) {  // 2 strips of 2P rows
+    case 1: wave_smooth_rb<T, 32, 1>(p, rhs, fl, H, W, alpha, omega, npass); return;
+    case 2: wave_smooth_rb<T, 32, 2>(p, rhs, fl, H, W, alpha, omega, npass); return;
+    case 3: wave_smooth_rb<T, 32, 3>(p, rhs, fl, H, W, alpha, omega, npass); return;
+    case 4: wave_smooth_rb<T, 32, 4>(p, rhs, fl, H, W, alpha, omega, npass); return;
+    case 5: wave_smooth_rb<T, 32, 5>(p, rhs, fl, H, W, alpha, omega, npass); return;
+    case 6: wave_smooth_rb<T, 32, 6>(p, rhs, fl, H, W, alpha, omega, npass); return;
+    case 7: wave_smooth_rb<T, 32, 7>(p, rhs, fl, H, W, alpha, omega, npass); return;
+    default: wave_smooth_rb<T, 32, 8>(p, rhs, fl, H, W, alpha, omega, npass); return;
+  }
+}
+
+template <typename T, int MODE, int NT, int WAVE>
 __device__ __forceinline__ void small_smooth(T* p, T* tmp, const T* rhs, const uint8_t* fl, int H, int W,
                              double alpha, double omega, int npass) {
+  if (MODE == MODE_RB && (WAVE & WAVE_LEVELS) && W <= 32 && H <= 32) {
+    if (threadIdx.x < 64) wave_smooth_level<T>(p, rhs, fl, H, W, alpha, omega, npass);
+    __syncthreads();
+    return;
+  }
   if (MODE == MODE_RB && W <= 64 && H <= 64) {
     // pairs per wave as a compile-time count, so no pass runs predicated-off
     // work for pairs the level does not have
@@ -1309,7 +1584,11 @@ __device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t*
     }
 }
 
-template <typename T, int MODE, int NT>
+// WAVE (bits): WAVE_COARSEST runs a red-black coarsest level of at most
+// 16 x 16 cells with wave_smooth_level instead of coarsest_wave;
+// WAVE_LEVELS also runs every red-black level of at most 32 x 32 cells, the
+// coarsest included, on wavefront 0 while the others wait at the barrier.
+template <typename T, int MODE, int NT, int WAVE>
 __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
   extern __shared__ __align__(16) unsigned char smem[];
   if (a.gate != nullptr && *a.gate == 0) return;
@@ -1390,8 +1669,8 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
   // down
   for (int l = 0; l < last; ++l) {
     const int H = a.H[l], W = a.W[l];
-    small_smooth<T, MODE, NT>(SP(l), tmp, SB(l), SF(l), H, W, a.alpha[l], a.omega[l],
-                              a.npass_smooth);
+    small_smooth<T, MODE, NT, WAVE>(SP(l), tmp, SB(l), SF(l), H, W, a.alpha[l], a.omega[l],
+                                    a.npass_smooth);
     const int Hc = a.H[l + 1], Wc = a.W[l + 1];
     const int N = H * W;
     for (int k = threadIdx.x; k < Hc * Wc; k += NT) {
@@ -1421,14 +1700,21 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
     __syncthreads();
   }
   // coarsest
-  if (a.H[last] <= 16 && a.W[last] <= 16) {
+  if (MODE == MODE_RB &&
+      (((WAVE & WAVE_LEVELS) && a.H[last] <= 32 && a.W[last] <= 32) ||
+       ((WAVE & WAVE_COARSEST) && a.H[last] <= 16 && a.W[last] <= 16))) {
+    if (threadIdx.x < 64)
+      wave_smooth_level<T>(SP(last), SB(last), SF(last), a.H[last], a.W[last], a.alpha[last],
+                           a.omega[last], a.npass_coarsest);
+    __syncthreads();
+  } else if (a.H[last] <= 16 && a.W[last] <= 16) {
     if (threadIdx.x < 64)
       coarsest_wave<T, MODE>(SP(last), SB(last), SF(last), a.H[last], a.W[last], a.alpha[last],
                              a.omega[last], a.npass_coarsest);
     __syncthreads();
   } else {
-    small_smooth<T, MODE, NT>(SP(last), tmp, SB(last), SF(last), a.H[last], a.W[last],
-                              a.alpha[last], a.omega[last], a.npass_coarsest);
+    small_smooth<T, MODE, NT, false>(SP(last), tmp, SB(last), SF(last), a.H[last], a.W[last],
+                                     a.alpha[last], a.omega[last], a.npass_coarsest);
   }
   // up
   for (int l = last - 1; l >= 0; --l) {
@@ -1442,8 +1728,8 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
       SP(l)[o] = SP(l)[o] + corr;
     }
     __syncthreads();
-    small_smooth<T, MODE, NT>(SP(l), tmp, SB(l), SF(l), H, W, a.alpha[l], a.omega[l],
-                              a.npass_smooth);
+    small_smooth<T, MODE, NT, WAVE>(SP(l), tmp, SB(l), SF(l), H, W, a.alpha[l], a.omega[l],
+                                    a.npass_smooth);
   }
   const int n0 = a.H[0] * a.W[0];
   for (int o = threadIdx.x; o < n0; o += NT) a.p_top_out[o] = SP(0)[o];
@@ -1452,9 +1738,9 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
 #undef SF
 }
 
-template <typename T, int MODE, int NT>
+template <typename T, int MODE, int NT, int WAVE = 0>
 __global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
-  vcycle_small_body<T, MODE, NT>(a);
+  vcycle_small_body<T, MODE, NT, WAVE>(a);
 }
 
 constexpr int NB_X = 64, NB_Y = 4, NB_T = NB_X * NB_Y;
@@ -1742,15 +2028,17 @@ enum : int {
   NUM_VARIANTS = 10
 };
 // Defaults per dtype [f32?][finest, coarse, small coarse], from timing every
-// variant on the 2048^2 V(2,2) workload: float64 finest level 64 x 64 with p
-// staged by LDS-DMA (35 us; 39 us with p staged through registers, 41 us for
-// 64 x 48 all in LDS), float32 finest 64 x 64 with the rhs in registers,
-// coarse levels 64 x 32 in LDS.  Coarse levels of at most SMALL_TILED_CELLS
-// cells occupy a few CUs, so a launch lasts as long as one workgroup: 64 x 16
-// regions halve the work on that critical path (two cells per lane and pass
-// instead of four).
-static int g_variant[2][3] = {{VAR_DMA64, VAR_LDS32, VAR_LDS16},
-                              {VAR_LDS64R, VAR_LDS32, VAR_LDS16}};
+// variant on the launches of the 2048^2 V(2,2) workload (4 passes and the
+// restriction, or the prolongation and 4 passes): finest level 64 x 64 with p
+// in LDS and the rhs and flags in registers (float64 31 / 30 us against 34 /
+// 33 us with p staged by LDS-DMA; float32 27 / 28 us), the 1024^2 level
+// 64 x 32 likewise (float64 26 us for both launches against 28 us all in
+// LDS).  Coarse levels of at most SMALL_TILED_CELLS cells occupy a few CUs,
+// so a launch lasts as long as one workgroup: 64 x 16 regions all in LDS halve
+// the work on that critical path (two cells per lane and pass instead of
+// four).
+static int g_variant[2][3] = {{VAR_LDS64R, VAR_LDS32R, VAR_LDS16},
+                              {VAR_LDS64R, VAR_LDS32R, VAR_LDS16}};
 constexpr long SMALL_TILED_CELLS = 512L * 512L;
 
 inline int variant_rh(int v);
@@ -1765,7 +2053,11 @@ inline int pick_variant(bool f32, bool fine, int H, int W, int R) {
 
 // Threads of the single-workgroup small-level V-cycle, per dtype [f32?].
 static int g_small_nt[2] = {1024, 1024};
-constexpr int SMALL_NT_CHOICES[3] = {256, 512, 1024};
+// red-black levels of at most 32 x 32 cells (and the coarsest) on one
+// wavefront with DPP / permlane exchange, else the multi-wave LDS passes
+static int g_small_wave[2] = {0, 0};
+constexpr int SMALL_NT_CHOICES[8] = {256, 512, 1024, 256, 512, 256, 512, 1024};
+constexpr int SMALL_WAVE_CHOICES[8] = {0, 0, 0, 3, 3, 1, 1, 1};
 
 inline int variant_rh(int v) {
   switch (v) {
@@ -1902,7 +2194,12 @@ static size_t small_lds_limit() {
         reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 512>),
         reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, 512>),
         reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 1024>),
-        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, 1024>)};
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_JACOBI, 1024>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 256, 3>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 512, 3>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 256, 1>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 512, 1>),
+        reinterpret_cast<const void*>(&k_vcycle_small<T, MODE_RB, 1024, 1>)};
     bool ok = true;
     for (const void* f : fns)
       ok = ok && hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want) ==
@@ -1938,6 +2235,7 @@ int launch_small(int mode, const SmallArgs<T>& a, size_t lds, hipStream_t s) {
     ~Done() { prof_end(ev, sizeof(T) == 4 ? "vcycle_small_f32" : "vcycle_small_f64", bytes, s); }
   } done{ev, bytes, s};
   const int nt = g_small_nt[sizeof(T) == 4];
+  const int wave = mode == MODE_RB ? g_small_wave[sizeof(T) == 4] : 0;
 #define SMK_SMALL_LAUNCH(NTV)                                                                   \
   do {                                                                                         \
     if (mode == MODE_RB)                                                                       \
@@ -1945,7 +2243,20 @@ int launch_small(int mode, const SmallArgs<T>& a, size_t lds, hipStream_t s) {
     else                                                                                       \
       hipLaunchKernelGGL((k_vcycle_small<T, MODE_JACOBI, NTV>), dim3(1), dim3(NTV), lds, s, a); \
   } while (0)
-  if (nt == 1024)
+#define SMK_SMALL_WAVE(NTV, WV) \
+  hipLaunchKernelGGL((k_vcycle_small<T, MODE_RB, NTV, WV>), dim3(1), dim3(NTV), lds, s, a)
+  if (wave == 3 && nt == 512)
+    SMK_SMALL_WAVE(512, 3);
+  else if (wave == 3)
+    SMK_SMALL_WAVE(256, 3);
+  else if (wave == 1 && nt == 1024)
+    SMK_SMALL_WAVE(1024, 1);
+  else if (wave == 1 && nt == 512)
+    SMK_SMALL_WAVE(512, 1);
+  else if (wave == 1)
+    SMK_SMALL_WAVE(256, 1);
+#undef SMK_SMALL_WAVE
+  else if (nt == 1024)
     SMK_SMALL_LAUNCH(1024);
   else if (nt == 512)
     SMK_SMALL_LAUNCH(512);
@@ -3018,8 +3329,9 @@ int smk_set_smoother_variant(int dtype, int which, int variant) {
               "smk_set_smoother_variant: which is 0 (fine), 1 (coarse), 2 (small levels) or "
               "3 (coarse levels of at most 512x512 cells)");
   if (which == 2) {
-    SMK_REQUIRE(variant >= 0 && variant < 3, "smk_set_smoother_variant: bad variant %d", variant);
+    SMK_REQUIRE(variant >= 0 && variant < 8, "smk_set_smoother_variant: bad variant %d", variant);
     g_small_nt[dtype == SMK_F32] = SMALL_NT_CHOICES[variant];
+    g_small_wave[dtype == SMK_F32] = SMALL_WAVE_CHOICES[variant];
     return SMK_OK;
   }
   SMK_REQUIRE(variant >= 0 && variant < NUM_VARIANTS, "smk_set_smoother_variant: bad variant %d",
@@ -3032,7 +3344,10 @@ int smk_get_smoother_variant(int dtype, int which) {
   if ((dtype != SMK_F64 && dtype != SMK_F32) || which < 0 || which > 3) return -1;
   if (which == 2) {
     const int nt = g_small_nt[dtype == SMK_F32];
-    return nt == 1024 ? 2 : (nt == 512 ? 1 : 0);
+    const int wave = g_small_wave[dtype == SMK_F32];
+    for (int v = 0; v < 8; ++v)
+      if (SMALL_NT_CHOICES[v] == nt && SMALL_WAVE_CHOICES[v] == wave) return v;
+    return -1;
   }
   return g_variant[dtype == SMK_F32][which == 3 ? 2 : which];
 }

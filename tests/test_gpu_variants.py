@@ -81,10 +81,10 @@ class variant:
         _lib.call("smk_set_smoother_variant", self.code, _lib._WHICH[self.which], self.prev)
 
 
-def oracle_mg():
-    d, fmu, fmv = problem()
+def oracle_mg(H=H, W=W, **extra):
+    d, fmu, fmv = problem(H, W)
     ref = np.zeros((H, W))
-    O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **MG)
+    O.multigrid_solver(ref, d, fmu, fmv, 1.0, 1.0 / H, 0.005, **dict(MG, **extra))
     return ref
 
 
@@ -126,12 +126,22 @@ def test_tile_variant_rbsor(dtype, name):
     np.testing.assert_array_equal(got, want)
 
 
+# (H, W, levels): the small-level kernel then covers, from its top level down,
+# 25x17; 32x32, 16x16; 32x16, 16x8; 16x32, 8x16; 24x20, 12x10, 6x5 -- the
+# one-wavefront smoother's strips of 16 lanes (levels up to 16x16) and of 32
+# lanes (up to 32x32), with 1 to 8 row pairs per lane
+SMALL_SHAPES = [(200, 136, 5), (256, 256, 5), (128, 64, 4), (64, 128, 4), (48, 40, 4)]
+
+
+@pytest.mark.parametrize("shape", SMALL_SHAPES, ids=lambda s: "%dx%d" % s[:2])
 @pytest.mark.parametrize("name", SMALL_VARIANTS)
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
-def test_small_variant_multigrid(dtype, name):
-    want = oracle_mg() if dtype == "float64" else run_mg(dtype)[0]
+def test_small_variant_multigrid(dtype, name, shape):
+    h, w, levels = shape
+    kw = dict(num_levels=levels)
+    want = oracle_mg(h, w, **kw) if dtype == "float64" else run_mg(dtype, h, w, **kw)[0]
     with variant(dtype, "small", name):
-        got, _ = run_mg(dtype)
+        got, _ = run_mg(dtype, h, w, **kw)
     np.testing.assert_array_equal(got, want)
 
 
