@@ -355,13 +355,18 @@ __device__ __forceinline__ void rtile_issue(const SmoothArgs<T>& a, int bx, int 
     }
 }
 
-// Zero what lies outside the grid, and place p and the flags in LDS and the
+// Apply the prolonged correction (spatch: the staged coarse patch, or null),
+// zero what lies outside the grid, and place p and the flags in LDS and the
 // rhs and flags in the lane's registers.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ T prolong_at(const SmoothArgs<T>& a, const T* spatch, int bx, int by,
+                                        int gx, int gy);
+
 template <typename T, int RH, int NT, int KH>
 __device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int by, int h0,
                                              T (&pv)[KH][2], const T (&rv)[KH][2],
                                              const uint32_t (&fv)[KH][2], T* sp, uint8_t* sf,
-                                             RTile<T, RH, NT>& t) {
+                                             RTile<T, RH, NT>& t, const T* spatch) {
   using RT = RTile<T, RH, NT>;
   constexpr uint32_t F_ALL = F_FLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
   const int R = a.R;
@@ -370,6 +375,16 @@ __device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int
   const int px = lane & 1;
   const int gx = bx * TX - R + lane, gy0 = by * TY - R;
   const bool colin = gx >= 0 && gx < a.W;
+  if (spatch != nullptr) {  // the prolonged correction, on fluid cells
+#pragma unroll
+    for (int k = 0; k < KH; ++k)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
+        const T corr = prolong_at<T, RH, NT>(a, spatch, bx, by, gx, gy);
+        pv[k][q] = (fv[k][q] & F_FLUID) ? pv[k][q] + corr : pv[k][q];
+      }
+  }
 #pragma unroll
   for (int k = 0; k < KH; ++k) {
     uint32_t f[2];
@@ -406,31 +421,6 @@ __device__ __forceinline__ void rtile_reset(RTile<T, RH, NT>& t) {
     for (int q = 0; q < RTile<T, RH, NT>::KW; ++q) t.fw[c][q] = 0u;
 }
 
-// The stencil of the lane's colour-c cell of pair k at LDS offset o; returns
-// the neighbour sum with closed faces replaced by the centre value.
-template <typename T, int RH, int NT>
-__device__ __forceinline__ T rtile_nsum(const RTile<T, RH, NT>& t, int c, int k, const T* sp,
-                                        int o, T& ce, bool& fl) {
-  constexpr int RW = RTile<T, RH, NT>::RW;
-  ce = sp[o];
-  const T lv = sp[o - 1], rv = sp[o + 1], dv = sp[o - RW], uv = sp[o + RW];
-  if ((t.open[c] >> k) & 1u) {
-    fl = true;
-    return lv + rv + dv + uv;
-  }
-  const uint32_t* fwc = t.fw[c];
-  const T l = fbit(fwc, k, F_LEFT) ? lv : ce;
-  const T r = fbit(fwc, k, F_RIGHT) ? rv : ce;
-  const T d = fbit(fwc, k, F_DOWN) ? dv : ce;
-  const T u = fbit(fwc, k, F_UP) ? uv : ce;
-  fl = fbit(fwc, k, F_FLUID);
-  return l + r + d + u;
-}
-
-// One colour pass over the lane's cells of colour C (a compile-time colour
-// keeps every register array index constant).  Cells of one colour never
-// neighbour each other: every new value of the lane is computed before any
-// is stored.
 // OPEN: every pair of the lane's colour-C cells is open in every lane (no
 // selects), decided once per pass rather than per cell, so the pass is one
 // straight run of code.  The LDS reads of a batch of KB cells are issued
@@ -535,17 +525,33 @@ __device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTi
 
   T res[2][K];
   double acc = 0.0;
+  constexpr uint32_t ALL = (1u << K) - 1u;
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const int par = (lane + c) & 1;
+    // as in rtile_sweep_k: one wave-uniform decision for the colour's pairs
+    const bool open_all = (t.open[c] & ALL) == ALL;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int ly = 2 * (wave + k * NW) + par;
       const bool win = xout && ly >= R && ly < R + TY && gy0 + ly < H;
       const int o = win ? ly * RW + lane : RW + 1;
-      T ce;
+      const T ce = sp[o];
+      const T lv = sp[o - 1], rv = sp[o + 1], dv = sp[o - RW], uv = sp[o + RW];
+      T ns;
       bool fl;
-      const T ns = rtile_nsum(t, c, k, sp, o, ce, fl);
+      if (open_all) {
+        ns = lv + rv + dv + uv;
+        fl = true;
+      } else {
+        const uint32_t* fwc = t.fw[c];
+        const T l = fbit(fwc, k, F_LEFT) ? lv : ce;
+        const T r = fbit(fwc, k, F_RIGHT) ? rv : ce;
+        const T d = fbit(fwc, k, F_DOWN) ? dv : ce;
+        const T u = fbit(fwc, k, F_UP) ? uv : ce;
+        ns = l + r + d + u;
+        fl = fbit(fwc, k, F_FLUID);
+      }
       const double rr = residual_value(ce, ns, t.rb[c][k], a.alpha);
       res[c][k] = (T)rr;
       acc += (win && fl) ? rr * rr : 0.0;
@@ -637,48 +643,53 @@ __device__ __forceinline__ void cpatch_store(const T (&v)[CPatch<T, RH, NT>::PER
   }
 }
 
-// The prolonged correction added to the lane's fluid cells in LDS (the p of
-// the region is staged; the patch is in LDS; a barrier follows).  The
-// sample position of fine index g is g / 2 - 0.25, never an integer, so
+// The bilinear prolongation of the staged coarse patch at fine cell (gx, gy).
+// The sample position of fine index g is g / 2 - 0.25, never an integer, so
 // sample_field's floor and ceil are (g - 1) >> 1 and one more; with the same
-// clamping, weights (x minus the clamped floor, exact) and lerps the values
-// are sample_field's.  The column part is the lane's, the row part the pair's.
+// clamping, weights (x minus the clamped floor, exact) and lerps the value is
+// sample_field's.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ T prolong_at(const SmoothArgs<T>& a, const T* spatch, int bx, int by,
+                                        int gx, int gy) {
+  constexpr int PW = CPatch<T, RH, NT>::PW;
+  const int R = a.R, TX = 64 - 2 * R, TY = RH - 2 * R;
+  const int cx0 = (bx * TX - R - 1) >> 1, cy0 = (by * TY - R - 1) >> 1;
+  const int xf = (gx - 1) >> 1, yf = (gy - 1) >> 1;
+  const int x0 = min(max(xf, 0), a.Wc - 1), x1 = min(max(xf + 1, 0), a.Wc - 1);
+  const int y0 = min(max(yf, 0), a.Hc - 1), y1 = min(max(yf + 1, 0), a.Hc - 1);
+  const double tx = ((gx + 0.5) * 0.5 - 0.5) - (double)x0;
+  const double ty = ((gy + 0.5) * 0.5 - 0.5) - (double)y0;
+  const int r0 = (y0 - cy0) * PW, r1 = (y1 - cy0) * PW;
+  const double c00 = (double)spatch[r0 + x0 - cx0], c10 = (double)spatch[r0 + x1 - cx0];
+  const double c01 = (double)spatch[r1 + x0 - cx0], c11 = (double)spatch[r1 + x1 - cx0];
+  const double c0 = c00 + (c10 - c00) * tx;
+  const double c1 = c01 + (c11 - c01) * tx;
+  return (T)(c0 + (c1 - c0) * ty);
+}
+
+// The prolonged correction added to the lane's fluid cells in LDS (the p of
+// the region is staged; the patch is in LDS; a barrier follows).
 template <typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_prolong(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
                                               int bx, int by, T* sp, const T* spatch) {
   using RT = RTile<T, RH, NT>;
-  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW, PW = CPatch<T, RH, NT>::PW;
+  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
   const int R = a.R;
   const int TX = RW - 2 * R, TY = RH - 2 * R;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int gx = bx * TX - R + lane, gy0 = by * TY - R;
-  const int cx0 = (bx * TX - R - 1) >> 1, cy0 = (gy0 - 1) >> 1;
   const int px = lane & 1;
-  const int xf = (gx - 1) >> 1;
-  const int x0 = min(max(xf, 0), a.Wc - 1), x1 = min(max(xf + 1, 0), a.Wc - 1);
-  const double tx = ((gx + 0.5) * 0.5 - 0.5) - (double)x0;
-  const T* col0 = spatch + (x0 - cx0);
-  const T* col1 = spatch + (x1 - cx0);
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int ly = 2 * (wave + k * NW) + q;
-      const int gy = gy0 + ly;
-      const int yf = (gy - 1) >> 1;
-      const int y0 = min(max(yf, 0), a.Hc - 1), y1 = min(max(yf + 1, 0), a.Hc - 1);
-      const double ty = ((gy + 0.5) * 0.5 - 0.5) - (double)y0;
-      const int r0 = (y0 - cy0) * PW, r1 = (y1 - cy0) * PW;
-      const double c00 = (double)col0[r0], c10 = (double)col1[r0];
-      const double c01 = (double)col0[r1], c11 = (double)col1[r1];
-      const double c0 = c00 + (c10 - c00) * tx;
-      const double c1 = c01 + (c11 - c01) * tx;
-      const T corr = (T)(c0 + (c1 - c0) * ty);
+      const T corr = prolong_at<T, RH, NT>(a, spatch, bx, by, gx, gy0 + ly);
       // colour of row 2m + q is (q + x) & 1; select the word, not the array
       const uint32_t fwd = ((q + px) & 1) ? t.fw[1][k >> 2] : t.fw[0][k >> 2];
       const bool fl = ((fwd >> (8 * (k & 3))) & F_FLUID) != 0u;
       const int o = ly * RW + lane;
-      if (fl) sp[o] = sp[o] + corr;
+      sp[o] = fl ? sp[o] + corr : sp[o];
     }
 }
 
@@ -706,16 +717,16 @@ __device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int 
     T pv[KH][2], rv[KH][2];
     uint32_t fv[KH][2];
     rtile_issue<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv);
-    if (h0 == 0 && prolong) cpatch_store<T, RH, NT>(cv, spatch);
-    rtile_commit<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv, sp, sf, t);
+    if (h0 == 0 && prolong) {  // the whole patch in LDS before any lane samples it
+      cpatch_store<T, RH, NT>(cv, spatch);
+      __syncthreads();
+    }
+    rtile_commit<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv, sp, sf, t,
+                                prolong ? spatch : nullptr);
   }
   t.open[0] = __builtin_amdgcn_readfirstlane(t.open[0]);
   t.open[1] = __builtin_amdgcn_readfirstlane(t.open[1]);
   __syncthreads();
-  if (prolong) {
-    rtile_prolong<T, RH, NT>(a, t, bx, by, sp, spatch);
-    __syncthreads();
-  }
   rtile_passes(a, t, sp);
   rtile_epilogue(a, t, bx, by, part, sp, sf, red);
 }
