@@ -164,13 +164,14 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30) 
 
 
 def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3,
-                inner: int = 1):
+                inner: int = 1, coarsest: str = "sweeps"):
     import torch
 
     from cfd_amd import scenes
 
     sim = scenes.make_simulation(scene, dtype="float64", device="cuda",
-                                 solver_overrides={"precision": precision, "inner_cycles": inner})
+                                 solver_overrides={"precision": precision, "inner_cycles": inner,
+                                                   "coarsest_solve": coarsest})
     torch.cuda.synchronize()
     for _ in range(warmup):
         sim.substep(scene.dt)
@@ -442,6 +443,9 @@ def main():
     ap.add_argument("--variants", default="fp64,mixed")
     ap.add_argument("--mixed-inner", type=int, default=2,
                     help="FP32 V-cycles per FP64 refinement in the mixed variant")
+    ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator"], default="operator",
+                    help="coarsest level of the mixed variant's FP32 V-cycles: the 50 sweeps, "
+                         "or those sweeps applied as their precomputed 256x256 operator")
     ap.add_argument("--batched-scenes", type=int, default=96,
                     help="scenes of the uneven-batch workload (0 skips it)")
     ap.add_argument("--diff-steps", type=int, default=64,
@@ -494,9 +498,11 @@ def main():
     for var in args.variants.split(","):
         log(f"variant {var}: warmup {args.warmup}, steps {args.steps}")
         inner = args.mixed_inner if var == "mixed" else 1
+        coarsest = args.mixed_coarsest if var == "mixed" else "sweeps"
         r, _ = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
-                           args.warmup, inner=inner)
+                           args.warmup, inner=inner, coarsest=coarsest)
         r["inner_cycles_per_refinement"] = inner
+        r["coarsest_solve"] = coarsest
         r["alg_bytes_per_cell_step"] = algorithmic_bytes_per_cell(
             r["v_cycles_per_step"], var == "mixed", r["levels"] or 8, r["refinements_per_step"])
         r["alg_GBps_whole_step"] = r["alg_bytes_per_cell_step"] * r["cell_steps_per_s"] / 1e9
@@ -569,7 +575,8 @@ def main():
         "config": {"workload": f"plume_{args.N}_mg", "grid": [args.N, args.N],
                    "solver": "multigrid V(2,2) RB-GS, 8 levels, coarsest 50 sweeps, tol 1e-8, "
                              "rediscretized coarse scale",
-                   "variant": main_var},
+                   "variant": main_var,
+                   "coarsest_solve": results[main_var].get("coarsest_solve", "sweeps")},
         "variants": results,
         "roofline": None if dom is None else dict(
             roofline_of(dom), whole_step_alg_GBps=m["alg_GBps_whole_step"],

@@ -212,6 +212,13 @@ typedef struct smk_mg smk_mg;
 #define SMK_PREC_NATIVE 0 /* V-cycles in the handle's dtype               */
 #define SMK_PREC_MIXED 1  /* FP32 V-cycles inside FP64 defect correction  */
 
+#define SMK_COARSEST_SWEEPS 0   /* coarsest level: coarsest_iterations sweeps */
+#define SMK_COARSEST_OPERATOR 1 /* mixed precision, red-black, a coarsest level
+                                   of at most 256 cells: the FP32 V-cycles apply
+                                   those sweeps as their precomputed matrix
+                                   (the sweeps from a zero guess are linear in
+                                   the rhs); other solves use the sweeps     */
+
 typedef struct smk_mg_params {
   int v_cycles;            /* fixed cycles (tolerance <= 0) or the cap    */
   int smoother;            /* SMK_SMOOTH_*  (MultigridParams.smoother_type) */
@@ -226,6 +233,7 @@ typedef struct smk_mg_params {
   double coarsest_omega;   /* coarsest level only: 0 = as smoother_omega
                               (reference); > 0 forces it; < 0 the optimal
                               SOR omega of the coarsest level's size (new) */
+  int coarsest_solve;      /* SMK_COARSEST_* (new; 0 = the reference's)    */
 } smk_mg_params;
 
 /* multigrid/solver.py:34-92 (_build_levels, _build_buffers, _ensure_cache):

@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "lib", "libsmoke_hip.so"))
 
-ABI_VERSION = 2  # smk_abi_version() of the library these bindings describe
+ABI_VERSION = 3  # smk_abi_version() of the library these bindings describe
 
 SMK_F64 = 0
 SMK_F32 = 1
@@ -26,6 +26,8 @@ SMK_COARSE_REFERENCE = 0
 SMK_COARSE_REDISCRETIZED = 1
 SMK_PREC_NATIVE = 0
 SMK_PREC_MIXED = 1
+SMK_COARSEST_SWEEPS = 0
+SMK_COARSEST_OPERATOR = 1
 SMK_ADJ_GATHER = 0
 SMK_ADJ_SCATTER = 1
 
@@ -70,6 +72,7 @@ class MgParams(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("inner_cycles", ctypes.c_int),
         ("coarsest_omega", ctypes.c_double),
+        ("coarsest_solve", ctypes.c_int),
     ]
 
 

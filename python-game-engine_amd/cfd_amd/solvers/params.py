@@ -70,6 +70,9 @@ class MultigridParams:
     precision: str = "native"  # new
     inner_cycles: int = 1  # new
     coarsest_omega: float = 0.0  # new: 0 = smoother_omega; < 0 optimal for the coarsest size
+    # new: "sweeps" (the reference's), or "operator": the mixed solver's FP32
+    # V-cycles apply the coarsest sweeps as their precomputed matrix
+    coarsest_solve: str = "sweeps"
 
     def string(self):
         return (f"MultigridParams_smoother_{self.smoother_type}_{self.smoother_backend}"

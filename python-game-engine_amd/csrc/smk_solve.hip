@@ -1200,6 +1200,7 @@ struct SmallArgs {
   int zero_top;
   int npass_smooth;    // per pre/post smoothing (RB: 2*sweeps)
   int npass_coarsest;  // coarsest level
+  const float* cop;    // red-black coarsest level by its precomputed operator (or null)
   const int* gate;     // nothing is done while *gate == 0
 };
 
@@ -1595,6 +1596,65 @@ __device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t*
     }
 }
 
+// x = op b for a coarsest level of n <= 256 cells: op[j * n + i] is the
+// result in cell i of the coarsest red-black passes from a zero guess with
+// rhs = unit vector j (k_coarse_op).  The passes are linear in the rhs, so
+// this is their result for b up to rounding.  Thread t covers rows
+// 4 (t % 64) .. + 3 of the columns j = t / 64 (mod NT / 64): every load of
+// the matrix is a float4 issued before any is used (one memory latency for
+// the whole product, where a column-by-column loop waits once per column);
+// the NT / 64 partial sums meet in LDS.
+template <typename T, int NT>
+__device__ __forceinline__ void coarsest_apply_op(const float* __restrict__ op, const T* b, T* x,
+                                                  int n) {
+  constexpr int G = NT / 64;            // column groups
+  constexpr int JMAX = (256 + G - 1) / G;  // columns per group at most
+  __shared__ float part[G][256];
+  const int q = threadIdx.x % 64, g = threadIdx.x / 64;
+  const int i0 = 4 * q;
+  if ((n & 3) == 0) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 m[JMAX];
+    float bj[JMAX];
+#pragma unroll
+    for (int k = 0; k < JMAX; ++k) {
+      const int j = g + k * G;
+      const bool ok = j < n && i0 < n;
+      m[k] = ok ? *reinterpret_cast<const float4*>(op + (size_t)j * n + i0)
+                : make_float4(0.f, 0.f, 0.f, 0.f);
+      bj[k] = j < n ? (float)b[j] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < JMAX; ++k) {
+      acc.x += m[k].x * bj[k];
+      acc.y += m[k].y * bj[k];
+      acc.z += m[k].z * bj[k];
+      acc.w += m[k].w * bj[k];
+    }
+    part[g][i0] = acc.x;
+    part[g][i0 + 1] = acc.y;
+    part[g][i0 + 2] = acc.z;
+    part[g][i0 + 3] = acc.w;
+  } else {  // rows one at a time
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + r;
+      float acc = 0.f;
+      for (int j = g; j < n; j += G)
+        if (i < n) acc += op[(size_t)j * n + i] * (float)b[j];
+      part[g][i] = acc;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < n) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < G; ++k) acc += (double)part[k][threadIdx.x];
+    x[threadIdx.x] = (T)acc;
+  }
+  __syncthreads();
+}
+
 // WAVE (bits): WAVE_COARSEST runs a red-black coarsest level of at most
 // 16 x 16 cells with wave_smooth_level instead of coarsest_wave;
 // WAVE_LEVELS also runs every red-black level of at most 32 x 32 cells, the
@@ -1711,7 +1771,9 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
     __syncthreads();
   }
   // coarsest
-  if (MODE == MODE_RB &&
+  if (MODE == MODE_RB && a.cop != nullptr) {
+    coarsest_apply_op<T, NT>(a.cop, SB(last), SP(last), a.H[last] * a.W[last]);
+  } else if (MODE == MODE_RB &&
       (((WAVE & WAVE_LEVELS) && a.H[last] <= 32 && a.W[last] <= 32) ||
        ((WAVE & WAVE_COARSEST) && a.H[last] <= 16 && a.W[last] <= 16))) {
     if (threadIdx.x < 64)
@@ -1752,6 +1814,26 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
 template <typename T, int MODE, int NT, int WAVE = 0>
 __global__ __launch_bounds__(NT) void k_vcycle_small(const SmallArgs<T> a) {
   vcycle_small_body<T, MODE, NT, WAVE>(a);
+}
+
+// Column j of the float32 coarsest-level operator (coarsest_apply_op): the
+// red-black passes of coarsest_wave, from p = 0 with rhs = unit vector j, on
+// one wavefront per column.
+__global__ __launch_bounds__(64) void k_coarse_op(const uint8_t* __restrict__ flags, int H, int W,
+                                                  double alpha, double omega, int npass,
+                                                  float* __restrict__ op) {
+  __shared__ float p[256], rhs[256];
+  __shared__ uint8_t fl[256];
+  const int n = H * W, j = blockIdx.x;
+  for (int i = threadIdx.x; i < 256; i += 64) {
+    p[i] = 0.0f;
+    rhs[i] = i == j ? 1.0f : 0.0f;
+    fl[i] = i < n ? flags[i] : (uint8_t)0;
+  }
+  __syncthreads();
+  coarsest_wave<float, MODE_RB>(p, rhs, fl, H, W, alpha, omega, npass);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 64) op[(size_t)j * n + i] = p[i];
 }
 
 constexpr int NB_X = 64, NB_Y = 4, NB_T = NB_X * NB_Y;
@@ -2773,6 +2855,11 @@ struct smk_mg {
   unsigned ring_seq;
   int small_from[2];  // first level handled by k_vcycle_small: [native dtype, float32]
   int small_mode[2];  // smoother mode small_from was computed for
+  // float32 coarsest-level operator (SMK_COARSEST_OPERATOR) and what it was
+  // built for
+  float* cop = nullptr;
+  double cop_alpha = 0.0, cop_omega = 0.0;
+  int cop_npass = -1;
 };
 
 namespace smk {
@@ -2791,6 +2878,7 @@ struct MgRun {
   int si, ci;
   int rediscretize;
   double rho, dt;
+  const float* cop;  // float32 coarsest-level operator for the small kernel (or null)
 };
 
 inline double level_omega(const MgRun& run, const Level& L, bool coarsest = false) {
@@ -2921,6 +3009,7 @@ int mg_small(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zer
   a.zero_top = zero_init ? 1 : 0;
   a.npass_smooth = run.mode == MODE_RB ? 2 * run.si : run.si;
   a.npass_coarsest = run.mode == MODE_RB ? 2 * run.ci : run.ci;
+  a.cop = (sizeof(T) == 4 && use32) ? run.cop : nullptr;
   a.gate = run.gate;
   const size_t lds = small_lds_bytes(a.H, a.W, a.nl, sizeof(T), run.mode);
   return launch_small<T>(run.mode, a, lds, s);
@@ -3185,6 +3274,36 @@ static int mg_ensure32(smk_mg* mg) {
   return SMK_OK;
 }
 
+// The float32 operator of the coarsest level's red-black passes (built once
+// per handle and alpha / omega / pass count); null when the coarsest level
+// has more than 256 cells (or the smoother is not red-black).
+inline int ensure_coarse_op(smk_mg* mg, const MgRun& run, double h, const float** out,
+                            hipStream_t s) {
+  *out = nullptr;
+  const Level& C = mg->lv[mg->nl - 1];
+  if (run.mode != MODE_RB || C.H > 16 || C.W > 16) return SMK_OK;
+  const double alpha = calc_alpha(h * (double)(1L << (mg->nl - 1)), run.rho, run.dt);
+  const double omega = level_omega(run, C, true);
+  const int npass = 2 * run.ci;
+  if (mg->cop == nullptr) {
+    int rc = mg_alloc(reinterpret_cast<void**>(&mg->cop), (size_t)256 * 256 * sizeof(float));
+    if (rc) return rc;
+    mg->cop_npass = -1;
+  }
+  if (mg->cop_npass != npass || mg->cop_alpha != alpha || mg->cop_omega != omega) {
+    const int n = C.H * C.W;
+    hipLaunchKernelGGL(k_coarse_op, dim3(n), dim3(64), 0, s, C.flags, C.H, C.W, alpha, omega,
+                       npass, mg->cop);
+    int rc = check_launch("k_coarse_op");
+    if (rc) return rc;
+    mg->cop_alpha = alpha;
+    mg->cop_omega = omega;
+    mg->cop_npass = npass;
+  }
+  *out = mg->cop;
+  return SMK_OK;
+}
+
 // FP32 V-cycles inside FP64 defect correction: r = alpha*div - A p in FP64
 // (with its norm), defect = r/alpha in FP32, e = k FP32 V-cycles from zero,
 // p += e in FP64; repeat until ||r||/||alpha*div|| <= tolerance.
@@ -3193,8 +3312,12 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
   int rc = mg_ensure32(mg);
   if (rc) return rc;
   Level& L0 = mg->lv[0];
-  const MgRun run = make_run(prm, rho, dt);
+  MgRun run = make_run(prm, rho, dt);
   mg_prepare_small<float>(mg, 1, run.mode);
+  if (prm->coarsest_solve == SMK_COARSEST_OPERATOR) {
+    rc = ensure_coarse_op(mg, run, h, &run.cop, s);
+    if (rc) return rc;
+  }
   const double alpha = calc_alpha(h, rho, dt);
   const int H = mg->H, W = mg->W;
   const long n = (long)H * W;
@@ -3475,6 +3598,7 @@ int smk_mg_destroy(smk_mg* mg) {
   (void)hipFree(mg->partial);
   (void)hipFree(mg->result);
   (void)hipFree(mg->d_state);
+  (void)hipFree(mg->cop);
   if (mg->h_ring) (void)hipHostFree(mg->h_ring);
   delete mg;
   return SMK_OK;
