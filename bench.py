@@ -164,14 +164,15 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30) 
 
 
 def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3,
-                inner: int = 1, coarsest: str = "sweeps"):
+                inner: int = 1, coarsest: str = "sweeps", arith: str = "float64"):
     import torch
 
     from cfd_amd import scenes
 
     sim = scenes.make_simulation(scene, dtype="float64", device="cuda",
                                  solver_overrides={"precision": precision, "inner_cycles": inner,
-                                                   "coarsest_solve": coarsest})
+                                                   "coarsest_solve": coarsest,
+                                                   "inner_arithmetic": arith})
     torch.cuda.synchronize()
     for _ in range(warmup):
         sim.substep(scene.dt)
@@ -446,6 +447,9 @@ def main():
     ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator"], default="operator",
                     help="coarsest level of the mixed variant's FP32 V-cycles: the 50 sweeps, "
                          "or those sweeps applied as their precomputed 256x256 operator")
+    ap.add_argument("--mixed-arith", choices=["float64", "float32"], default="float32",
+                    help="arithmetic of the mixed variant's FP32 V-cycle passes: the reference's "
+                         "float64 expression, or float32")
     ap.add_argument("--batched-scenes", type=int, default=96,
                     help="scenes of the uneven-batch workload (0 skips it)")
     ap.add_argument("--diff-steps", type=int, default=64,
@@ -499,10 +503,12 @@ def main():
         log(f"variant {var}: warmup {args.warmup}, steps {args.steps}")
         inner = args.mixed_inner if var == "mixed" else 1
         coarsest = args.mixed_coarsest if var == "mixed" else "sweeps"
+        arith = args.mixed_arith if var == "mixed" else "float64"
         r, _ = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
-                           args.warmup, inner=inner, coarsest=coarsest)
+                           args.warmup, inner=inner, coarsest=coarsest, arith=arith)
         r["inner_cycles_per_refinement"] = inner
         r["coarsest_solve"] = coarsest
+        r["inner_arithmetic"] = arith if var == "mixed" else None
         r["alg_bytes_per_cell_step"] = algorithmic_bytes_per_cell(
             r["v_cycles_per_step"], var == "mixed", r["levels"] or 8, r["refinements_per_step"])
         r["alg_GBps_whole_step"] = r["alg_bytes_per_cell_step"] * r["cell_steps_per_s"] / 1e9
@@ -576,7 +582,8 @@ def main():
                    "solver": "multigrid V(2,2) RB-GS, 8 levels, coarsest 50 sweeps, tol 1e-8, "
                              "rediscretized coarse scale",
                    "variant": main_var,
-                   "coarsest_solve": results[main_var].get("coarsest_solve", "sweeps")},
+                   "coarsest_solve": results[main_var].get("coarsest_solve", "sweeps"),
+                   "inner_arithmetic": results[main_var].get("inner_arithmetic")},
         "variants": results,
         "roofline": None if dom is None else dict(
             roofline_of(dom), whole_step_alg_GBps=m["alg_GBps_whole_step"],

@@ -212,6 +212,11 @@ typedef struct smk_mg smk_mg;
 #define SMK_PREC_NATIVE 0 /* V-cycles in the handle's dtype               */
 #define SMK_PREC_MIXED 1  /* FP32 V-cycles inside FP64 defect correction  */
 
+#define SMK_INNER_F64 0 /* mixed: the FP32 V-cycles evaluate the reference's
+                           float64 expression, as a float32 field's solve does */
+#define SMK_INNER_F32 1 /* mixed: their tile passes and residuals in float32
+                           arithmetic (the inner solve need only converge)   */
+
 #define SMK_COARSEST_SWEEPS 0   /* coarsest level: coarsest_iterations sweeps */
 #define SMK_COARSEST_OPERATOR 1 /* mixed precision, red-black, a coarsest level
                                    of at most 256 cells: the FP32 V-cycles apply
@@ -234,6 +239,7 @@ typedef struct smk_mg_params {
                               (reference); > 0 forces it; < 0 the optimal
                               SOR omega of the coarsest level's size (new) */
   int coarsest_solve;      /* SMK_COARSEST_* (new; 0 = the reference's)    */
+  int inner_arithmetic;    /* SMK_INNER_* (new; mixed precision only)       */
 } smk_mg_params;
 
 /* multigrid/solver.py:34-92 (_build_levels, _build_buffers, _ensure_cache):

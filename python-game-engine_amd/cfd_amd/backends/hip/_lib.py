@@ -26,6 +26,8 @@ SMK_COARSE_REFERENCE = 0
 SMK_COARSE_REDISCRETIZED = 1
 SMK_PREC_NATIVE = 0
 SMK_PREC_MIXED = 1
+SMK_INNER_F64 = 0
+SMK_INNER_F32 = 1
 SMK_COARSEST_SWEEPS = 0
 SMK_COARSEST_OPERATOR = 1
 SMK_ADJ_GATHER = 0
@@ -73,6 +75,7 @@ class MgParams(ctypes.Structure):
         ("inner_cycles", ctypes.c_int),
         ("coarsest_omega", ctypes.c_double),
         ("coarsest_solve", ctypes.c_int),
+        ("inner_arithmetic", ctypes.c_int),
     ]
 
 

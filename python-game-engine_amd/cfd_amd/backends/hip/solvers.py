@@ -17,7 +17,7 @@ import ctypes
 import torch
 
 from ._lib import (SMK_COARSE_REDISCRETIZED, SMK_COARSE_REFERENCE, SMK_COARSEST_OPERATOR,
-                   SMK_COARSEST_SWEEPS, SMK_PREC_MIXED,
+                   SMK_COARSEST_SWEEPS, SMK_INNER_F32, SMK_INNER_F64, SMK_PREC_MIXED,
                    SMK_PREC_NATIVE, SMK_SMOOTH_GS, SMK_SMOOTH_JACOBI, SMK_SMOOTH_RBSOR, MgParams,
                    SolveInfo,
                    call, lib, stream_of)
@@ -132,6 +132,7 @@ _SMOOTHERS = {"jacobi": SMK_SMOOTH_JACOBI, "gauss_seidel": SMK_SMOOTH_GS,
 _COARSE = {"reference": SMK_COARSE_REFERENCE, "rediscretized": SMK_COARSE_REDISCRETIZED}
 _PREC = {"native": SMK_PREC_NATIVE, "mixed": SMK_PREC_MIXED}
 _COARSEST = {"sweeps": SMK_COARSEST_SWEEPS, "operator": SMK_COARSEST_OPERATOR}
+_INNER = {"float64": SMK_INNER_F64, "float32": SMK_INNER_F32}
 
 
 class _MgHandle:
@@ -193,7 +194,7 @@ class MultigridSolver:
                  smoother_iterations=2, coarsest_iterations=50, rebuild_masks=False,
                  profiler=None, smoother_omega=0.0, coarse_scale="reference", tolerance=0.0,
                  precision="native", inner_cycles=1, coarsest_omega=0.0,
-                 coarsest_solve="sweeps", **solver_kwargs):
+                 coarsest_solve="sweeps", inner_arithmetic="float64", **solver_kwargs):
         _check(pressure, divergence, face_mask_u, face_mask_v)
         if smoother_type not in _SMOOTHERS:
             raise ValueError(
@@ -206,6 +207,8 @@ class MultigridSolver:
             raise ValueError("mixed precision needs float64 pressure/divergence")
         if coarsest_solve not in _COARSEST:
             raise ValueError(f"coarsest_solve must be one of {list(_COARSEST)}")
+        if inner_arithmetic not in _INNER:
+            raise ValueError(f"inner_arithmetic must be one of {list(_INNER)}")
         self._ensure(face_mask_u, face_mask_v, num_levels, pressure, rebuild_masks)
         prm = MgParams(v_cycles=int(v_cycles), smoother=_SMOOTHERS[smoother_type],
                        smoother_iterations=int(smoother_iterations),
@@ -213,7 +216,8 @@ class MultigridSolver:
                        smoother_omega=float(smoother_omega), coarse_scale=_COARSE[coarse_scale],
                        tolerance=float(tolerance), precision=_PREC[precision],
                        inner_cycles=int(inner_cycles), coarsest_omega=float(coarsest_omega),
-                       coarsest_solve=_COARSEST[coarsest_solve])
+                       coarsest_solve=_COARSEST[coarsest_solve],
+                       inner_arithmetic=_INNER[inner_arithmetic])
         info = SolveInfo()
         call("smk_mg_solve", self._handle.ptr, pressure.data_ptr(), divergence.data_ptr(),
              float(fluid_density), float(cell_size), float(dt), ctypes.byref(prm),

@@ -73,6 +73,10 @@ class MultigridParams:
     # new: "sweeps" (the reference's), or "operator": the mixed solver's FP32
     # V-cycles apply the coarsest sweeps as their precomputed matrix
     coarsest_solve: str = "sweeps"
+    # new: "float64" (the reference's expression, as a float32 field's solve
+    # evaluates it) or "float32": the mixed solver's FP32 V-cycles smooth in
+    # float32 arithmetic
+    inner_arithmetic: str = "float64"
 
     def string(self):
         return (f"MultigridParams_smoother_{self.smoother_type}_{self.smoother_backend}"

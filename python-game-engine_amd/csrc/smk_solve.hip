@@ -55,6 +55,10 @@ struct SmoothArgs {
   double coarse_div;
   // norm-out
   double* norm_partial;
+  // float fields only: the passes and residuals in float32 arithmetic (the
+  // inner V-cycles of a mixed solve, which need only converge) instead of
+  // the reference's float64 expression
+  int fast_math;
   // the launch does nothing while *gate == 0 (device-side convergence)
   const int* gate;
 };
@@ -100,6 +104,15 @@ __device__ __forceinline__ double relax_value(int mode, T ce, T nsum, T b, doubl
 template <typename T>
 __device__ __forceinline__ double residual_value(T ce, T nsum, T b, double alpha) {
   return ((double)nsum + alpha * (double)b) - 4.0 * (double)ce;
+}
+
+// relax_value / residual_value in float32 arithmetic (SmoothArgs::fast_math)
+__device__ __forceinline__ float relax_f(float ce, float ns, float b, float alpha, float omega) {
+  const float nw = __builtin_fmaf(alpha, b, ns) * 0.25f;
+  return __builtin_fmaf(omega, nw - ce, ce);
+}
+__device__ __forceinline__ float residual_f(float ce, float ns, float b, float alpha) {
+  return __builtin_fmaf(-4.0f, ce, __builtin_fmaf(alpha, b, ns));
 }
 
 // Region = RH rows x 64 columns staged in LDS; lane x of every wavefront
@@ -429,7 +442,7 @@ __device__ __forceinline__ void rtile_reset(RTile<T, RH, NT>& t) {
 // batch's new values are stored before the next batch reads: a cell's reads
 // are its own value and cells of the other colour, which the pass does not
 // write.
-template <int C, bool OPEN, typename T, int RH, int NT>
+template <int C, bool OPEN, bool FM, typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
                                               T* sp, int lo) {
   using RT = RTile<T, RH, NT>;
@@ -472,33 +485,46 @@ __device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTil
         ns = l + r + d + u;
         fl = fbit(fwc, k, F_FLUID);
       }
-      const T nv = (T)relax_value(0, ce[j], ns, t.rb[C][k], a.alpha, a.omega);
+      T nv;
+      if constexpr (FM && sizeof(T) == 4)
+        nv = relax_f(ce[j], ns, t.rb[C][k], (float)a.alpha, (float)a.omega);
+      else
+        nv = (T)relax_value(0, ce[j], ns, t.rb[C][k], a.alpha, a.omega);
       if (valid[j] && fl) sp[oo[j]] = nv;
     }
   }
 }
 
-template <int C, typename T, int RH, int NT>
+template <int C, bool FM, typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_sweep(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
                                             T* sp, int lo) {
   constexpr uint32_t ALL = (1u << RTile<T, RH, NT>::K) - 1u;
   if ((t.open[C] & ALL) == ALL)  // wave-uniform
-    rtile_sweep_k<C, true>(a, t, sp, lo);
+    rtile_sweep_k<C, true, FM>(a, t, sp, lo);
   else
-    rtile_sweep_k<C, false>(a, t, sp, lo);
+    rtile_sweep_k<C, false, FM>(a, t, sp, lo);
+}
+
+template <bool FM, typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_passes_m(const SmoothArgs<T>& a,
+                                               const RTile<T, RH, NT>& t, T* sp) {
+  for (int pass = 0; pass < a.npass; ++pass) {
+    if (((a.first_colour + pass) & 1) == 0)
+      rtile_sweep<0, FM>(a, t, sp, pass + 1);
+    else
+      rtile_sweep<1, FM>(a, t, sp, pass + 1);
+    __syncthreads();
+  }
 }
 
 // npass colour passes; the valid window shrinks by one cell per pass.
 template <typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_passes(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
                                              T* sp) {
-  for (int pass = 0; pass < a.npass; ++pass) {
-    if (((a.first_colour + pass) & 1) == 0)
-      rtile_sweep<0>(a, t, sp, pass + 1);
-    else
-      rtile_sweep<1>(a, t, sp, pass + 1);
-    __syncthreads();
-  }
+  if (sizeof(T) == 4 && a.fast_math)
+    rtile_passes_m<true>(a, t, sp);
+  else
+    rtile_passes_m<false>(a, t, sp);
 }
 
 // Write the tile interior of p; then the residuals of the lane's own cells
@@ -552,7 +578,10 @@ __device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTi
         ns = l + r + d + u;
         fl = fbit(fwc, k, F_FLUID);
       }
-      const double rr = residual_value(ce, ns, t.rb[c][k], a.alpha);
+      const double rr = (sizeof(T) == 4 && a.fast_math)
+                            ? (double)residual_f((float)ce, (float)ns, (float)t.rb[c][k],
+                                                 (float)a.alpha)
+                            : residual_value(ce, ns, t.rb[c][k], a.alpha);
       res[c][k] = (T)rr;
       acc += (win && fl) ? rr * rr : 0.0;
     }
@@ -2879,6 +2908,7 @@ struct MgRun {
   int rediscretize;
   double rho, dt;
   const float* cop;  // float32 coarsest-level operator for the small kernel (or null)
+  bool fast;         // float32 arithmetic in the float32 levels' tile passes
 };
 
 inline double level_omega(const MgRun& run, const Level& L, bool coarsest = false) {
@@ -2924,7 +2954,7 @@ template <typename T>
 int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_init,
               const T* coarse_e, int Hc, int Wc, T* coarse_rhs, double coarse_div,
               double* norm_partial, int* nparts, double alpha, double omega, int mode,
-              hipStream_t s, const int* gate = nullptr) {
+              hipStream_t s, const int* gate = nullptr, bool fast_inner = false) {
   Level& L = lv[l];
   void** pb = use32 ? L.p32 : L.p;
   int& cur = use32 ? L.cur32 : L.cur;
@@ -2959,6 +2989,7 @@ int mg_smooth(std::vector<Level>& lv, int l, bool use32, int npass, bool zero_in
         pick_variant(sizeof(T) == 4, l == 0, L.H, L.W, a.R) == VAR_REG64)
       a.R += a.R & 1;
     a.zero_init = (first && zero_init) ? 1 : 0;
+    a.fast_math = (use32 && fast_inner) ? 1 : 0;
     a.gate = gate;
     if (first && coarse_e != nullptr) {
       a.coarse_e = coarse_e;
@@ -3098,18 +3129,18 @@ int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool ze
   const double omega = level_omega(run, L, coarsest);
   if (coarsest)
     return mg_smooth<T>(lv, l, use32, npass, zero_init, nullptr, 0, 0, nullptr, 1.0,
-                        norm_partial, nparts, alpha, omega, run.mode, s, run.gate);
+                        norm_partial, nparts, alpha, omega, run.mode, s, run.gate, run.fast);
   Level& C = lv[l + 1];
   T* crhs = reinterpret_cast<T*>(use32 ? C.rhs32 : C.rhs);
   const double coarse_div = run.rediscretize ? alpha : calc_alpha(h * 2.0, run.rho, run.dt);
   int rc = mg_smooth<T>(lv, l, use32, npass, zero_init, nullptr, 0, 0, crhs, coarse_div, nullptr,
-                        nullptr, alpha, omega, run.mode, s, run.gate);
+                        nullptr, alpha, omega, run.mode, s, run.gate, run.fast);
   if (rc) return rc;
   rc = mg_vcycle<T>(mg, l + 1, use32, run, h * 2.0, true, nullptr, nullptr, s);
   if (rc) return rc;
   const T* ce = reinterpret_cast<const T*>(use32 ? C.p32[C.cur32] : C.p[C.cur]);
   return mg_smooth<T>(lv, l, use32, npass, false, ce, C.H, C.W, nullptr, 1.0, norm_partial,
-                      nparts, alpha, omega, run.mode, s, run.gate);
+                      nparts, alpha, omega, run.mode, s, run.gate, run.fast);
 }
 
 inline MgRun make_run(const smk_mg_params* prm, double rho, double dt) {
@@ -3318,6 +3349,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     rc = ensure_coarse_op(mg, run, h, &run.cop, s);
     if (rc) return rc;
   }
+  run.fast = prm->inner_arithmetic == SMK_INNER_F32;
   const double alpha = calc_alpha(h, rho, dt);
   const int H = mg->H, W = mg->W;
   const long n = (long)H * W;
