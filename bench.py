@@ -474,7 +474,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(it measures one GPU; nothing here runs across GPUs)", file=sys.stderr)
         sys.exit(2)
 

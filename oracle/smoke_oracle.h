@@ -131,7 +131,8 @@ enum { ORC_SMOOTH_JACOBI = 0, ORC_SMOOTH_GS = 1, ORC_SMOOTH_RBSOR = 2 };
 /* multigrid/solver.py:16-177.  One call = reference __call__ with a fresh
  * cache.  smoother_omega <= 0 reproduces the reference (the smoother picks
  * its automatic omega per level, solver.py:107); > 0 forces that omega for
- * the RB-SOR smoother (the BASELINE "RB-GS" setting uses 1.0).
+ * the RB-SOR smoother (plain red-black Gauss-Seidel, as the 2048^2 multigrid
+ * workload runs it, is 1.0).
  * rediscretize = 0 reproduces the reference coarse right-hand side
  * restrict(r)/alpha(2h) (solver.py:124-125); 1 uses restrict(r)/alpha(h),
  * the rediscretised coarse operator (new).

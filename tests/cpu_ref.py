@@ -1,6 +1,6 @@
 """Backend ``"cpu_ref"``: the oracle registered behind the same registries as
 the ``"hip"`` backend, for every operator key and every solver key including
-the serial ``gauss_seidel`` (SURVEY's boundary contract).  Test
+the serial ``gauss_seidel``.  Test
 infrastructure: the product package never registers or imports it."""
 from __future__ import annotations
 
