@@ -25,6 +25,7 @@
 // Python-float coefficients as float64).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -880,9 +881,14 @@ __device__ __forceinline__ void smooth_tile64d_body(const SmoothArgs<T>& a, int 
 }
 
 // waves per SIMD the register allocation must allow: 4 when a float64 lane
-// holds six or more row pairs (more would spill), 5 otherwise
+// holds six or more row pairs (more would spill); 8 for the float32 64-row
+// tiles (64 VGPRs: four 512-thread workgroups per CU instead of three, three
+// dwords spilled outside the pass loops; 24.6 -> 24.2 us at 2048^2); 5
+// otherwise
 template <typename T, int RH, int NT>
-constexpr int r_waves() { return (sizeof(T) == 8 && RH / (2 * (NT / 64)) >= 6) ? 4 : 5; }
+constexpr int r_waves() {
+  return (sizeof(T) == 8 && RH / (2 * (NT / 64)) >= 6) ? 4 : (sizeof(T) == 4 && RH == 64 ? 8 : 5);
+}
 
 template <typename T, int RH, int NT>
 __global__ __launch_bounds__(NT, (r_waves<T, RH, NT>())) void k_smooth_fine_r(const SmoothArgs<T> a) {
