@@ -240,6 +240,10 @@ typedef struct smk_mg_params {
                               SOR omega of the coarsest level's size (new) */
   int coarsest_solve;      /* SMK_COARSEST_* (new; 0 = the reference's)    */
   int inner_arithmetic;    /* SMK_INNER_* (new; mixed precision only)       */
+  int zero_guess;          /* 1: p's content is ignored and taken as 0 (the
+                              caller's p.zero_() of simulation.py:310 folded
+                              in; every cell of p is written); 0: p is the
+                              initial guess (reference)                    */
 } smk_mg_params;
 
 /* multigrid/solver.py:34-92 (_build_levels, _build_buffers, _ensure_cache):

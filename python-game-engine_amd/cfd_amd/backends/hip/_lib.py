@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "lib", "libsmoke_hip.so"))
 
-ABI_VERSION = 3  # smk_abi_version() of the library these bindings describe
+ABI_VERSION = 4  # smk_abi_version() of the library these bindings describe
 
 SMK_F64 = 0
 SMK_F32 = 1
@@ -76,6 +76,7 @@ class MgParams(ctypes.Structure):
         ("coarsest_omega", ctypes.c_double),
         ("coarsest_solve", ctypes.c_int),
         ("inner_arithmetic", ctypes.c_int),
+        ("zero_guess", ctypes.c_int),
     ]
 
 

@@ -133,6 +133,7 @@ _COARSE = {"reference": SMK_COARSE_REFERENCE, "rediscretized": SMK_COARSE_REDISC
 _PREC = {"native": SMK_PREC_NATIVE, "mixed": SMK_PREC_MIXED}
 _COARSEST = {"sweeps": SMK_COARSEST_SWEEPS, "operator": SMK_COARSEST_OPERATOR}
 _INNER = {"float64": SMK_INNER_F64, "float32": SMK_INNER_F32}
+_GUESS = {"given": 0, "zero": 1}
 
 
 class _MgHandle:
@@ -170,6 +171,12 @@ class MultigridSolver:
     sweep as an exact wavefront, slow: one workgroup per level) or
     "rb_sor_gauss_seidel" -- the three solvers the reference's registry
     holds (solver.py:164 looks the smoother up there).
+
+    ``initial_guess`` (new): "given" (the reference: ``pressure`` is the
+    initial guess) or "zero": its content is ignored and taken as zero, and
+    every cell is written -- the caller's ``pressure.zero_()`` before the
+    solve (simulation.py:310) folded in, with the initial residual taken from
+    the right-hand side's norm (the same numbers, bit for bit).
     """
 
     def __init__(self):
@@ -194,7 +201,8 @@ class MultigridSolver:
                  smoother_iterations=2, coarsest_iterations=50, rebuild_masks=False,
                  profiler=None, smoother_omega=0.0, coarse_scale="reference", tolerance=0.0,
                  precision="native", inner_cycles=1, coarsest_omega=0.0,
-                 coarsest_solve="sweeps", inner_arithmetic="float64", **solver_kwargs):
+                 coarsest_solve="sweeps", inner_arithmetic="float64", initial_guess="given",
+                 **solver_kwargs):
         _check(pressure, divergence, face_mask_u, face_mask_v)
         if smoother_type not in _SMOOTHERS:
             raise ValueError(
@@ -209,6 +217,8 @@ class MultigridSolver:
             raise ValueError(f"coarsest_solve must be one of {list(_COARSEST)}")
         if inner_arithmetic not in _INNER:
             raise ValueError(f"inner_arithmetic must be one of {list(_INNER)}")
+        if initial_guess not in _GUESS:
+            raise ValueError(f"initial_guess must be one of {list(_GUESS)}")
         self._ensure(face_mask_u, face_mask_v, num_levels, pressure, rebuild_masks)
         prm = MgParams(v_cycles=int(v_cycles), smoother=_SMOOTHERS[smoother_type],
                        smoother_iterations=int(smoother_iterations),
@@ -217,7 +227,8 @@ class MultigridSolver:
                        tolerance=float(tolerance), precision=_PREC[precision],
                        inner_cycles=int(inner_cycles), coarsest_omega=float(coarsest_omega),
                        coarsest_solve=_COARSEST[coarsest_solve],
-                       inner_arithmetic=_INNER[inner_arithmetic])
+                       inner_arithmetic=_INNER[inner_arithmetic],
+                       zero_guess=_GUESS[initial_guess])
         info = SolveInfo()
         call("smk_mg_solve", self._handle.ptr, pressure.data_ptr(), divergence.data_ptr(),
              float(fluid_density), float(cell_size), float(dt), ctypes.byref(prm),

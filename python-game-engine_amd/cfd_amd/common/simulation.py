@@ -283,10 +283,18 @@ class SmokeSimulation2D:
                          backend=self.operators_backend.forces)
 
     def _solve_pressure(self):
-        """simulation.py:308-312 (p zeroed, solved with self.dt)."""
-        self.pressure.zero_()
+        """simulation.py:308-312 (p zeroed, solved with self.dt).  The hip
+        multigrid solver takes the zeroing over (initial_guess="zero": it
+        reads no p, writes every cell and reuses the rhs norm as the initial
+        residual), with the same result."""
+        extra = {}
+        if (self.pressure_solver_method == "multigrid"
+                and Backend(self.pressure_solver_backend) == Backend.HIP):
+            extra["initial_guess"] = "zero"
+        else:
+            self.pressure.zero_()
         solve_pressure(self.pressure, self.divergence, self._face_mask_u, self._face_mask_v,
-                       self.fluid_density, self.cell_size, self.dt, self)
+                       self.fluid_density, self.cell_size, self.dt, self, **extra)
         self.last_solve_info = dict(hip_solvers.LAST_SOLVE_INFO)
 
     def _apply_pressure_gradient(self, dt, fuse_boundaries=False):

@@ -1914,7 +1914,7 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
 // on every cell (k_add_correction), then the defect of p_out exactly as
 // k_residual_norm computes it, with every corrected value formed as
 // p_in + e in the same rounding.  p_out must not alias p_in (neighbours read
-// p_in while other threads write p_out).
+// p_in while other threads write p_out).  p_in == nullptr: p_in is all zero.
 template <typename T>
 __global__ __launch_bounds__(NB_T) void k_correct_defect(const T* __restrict__ p_in,
                                                          const float* __restrict__ e,
@@ -1934,7 +1934,7 @@ __global__ __launch_bounds__(NB_T) void k_correct_defect(const T* __restrict__ p
   double acc = 0.0;
   if (i < W && j < H) {
     const size_t o = (size_t)j * W + i;
-    auto q = [&](size_t k) -> T { return p_in[k] + (T)e[k]; };
+    auto q = [&](size_t k) -> T { return (p_in != nullptr ? p_in[k] : (T)0) + (T)e[k]; };
     const uint8_t f = flags[o];
     const T ce = q(o);
     p_out[o] = ce;
@@ -1953,21 +1953,30 @@ __global__ __launch_bounds__(NB_T) void k_correct_defect(const T* __restrict__ p
   if (threadIdx.x == 0) partial[by * gridDim.x + bx] = tot;
 }
 
-// ||alpha*rhs||^2 partials over fluid cells (the p = 0 residual).
+// ||alpha*rhs||^2 partials over fluid cells (the p = 0 residual), laid out
+// as k_residual_norm's: for p = 0 both give the same partials, bit for bit.
+// defect (optional): the FP32 defect of p = 0 exactly as k_residual_norm
+// writes it.
 template <typename T>
 __global__ __launch_bounds__(NB_T) void k_rhs_norm(const T* __restrict__ rhs,
                                                    const uint8_t* __restrict__ flags, int H,
                                                    int W, double alpha,
-                                                   double* __restrict__ partial) {
+                                                   double* __restrict__ partial,
+                                                   float* __restrict__ defect) {
   __shared__ double red[NB_T / 64];
   const int i = blockIdx.x * NB_X + (threadIdx.x & (NB_X - 1));
   const int j = blockIdx.y * NB_Y + (threadIdx.x / NB_X);
   double acc = 0.0;
   if (i < W && j < H) {
     const size_t o = (size_t)j * W + i;
-    if (flags[o] & F_FLUID) {
+    const bool fl = (flags[o] & F_FLUID) != 0;
+    if (fl) {
       const double b = alpha * (double)rhs[o];
       acc = b * b;
+    }
+    if (defect != nullptr) {
+      const double rv = fl ? poisson_residual((T)0, (T)0, (T)0, (T)0, (T)0, rhs[o], alpha) : 0.0;
+      defect[o] = fl ? (float)(rv / alpha) : 0.0f;
     }
   }
   const double tot = block_sum<NB_T>(acc, red);
@@ -2555,9 +2564,11 @@ int residual_partials_dev(const T* p, const T* rhs, const uint8_t* flags, int H,
 
 template <typename T>
 int rhs_norm2_dev(const T* rhs, const uint8_t* flags, int H, int W, double alpha,
-                  double* partial, double* result, int slot, hipStream_t s) {
+                  double* partial, double* result, int slot, hipStream_t s,
+                  float* defect = nullptr) {
   dim3 g(cdiv(W, NB_X), cdiv(H, NB_Y));
-  hipLaunchKernelGGL(k_rhs_norm<T>, g, dim3(NB_T), 0, s, rhs, flags, H, W, alpha, partial);
+  hipLaunchKernelGGL(k_rhs_norm<T>, g, dim3(NB_T), 0, s, rhs, flags, H, W, alpha, partial,
+                     defect);
   int rc = check_launch("k_rhs_norm");
   if (rc) return rc;
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partial, (int)(g.x * g.y),
@@ -3227,7 +3238,20 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
   int cycles = 0;
   double rel = -1.0;
   bool converged = false;
-  if (prm->tolerance > 0.0 && gateable(mg, run, 0)) {
+  const long n = (long)mg->H * mg->W;
+  // zero guess: the gated loop below never reads p before writing it (the
+  // first V-cycle starts from zero, the initial check reuses the rhs norm's
+  // partials, which equal the p = 0 residual's); the other loops start from
+  // a zeroed p
+  const bool zg = prm->zero_guess != 0;
+  const bool gated = prm->tolerance > 0.0 && gateable(mg, run, 0);
+  if (zg && !gated) {
+    hipLaunchKernelGGL(k_fill_gated<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, (T)0,
+                       nullptr);
+    rc = check_launch("k_fill_gated");
+    if (rc) return rc;
+  }
+  if (gated) {
     // norm of the initial guess, then V-cycle + norm per gated iteration
     rc = rhs_norm2_dev<T>(div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 1, s);
     if (rc) return rc;
@@ -3240,7 +3264,8 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
     rc = run_gated(mg, prm->v_cycles + 1, [&](int k, RingSlot* slot, unsigned sq) -> int {
       int nparts = -1;
       int r = SMK_OK;
-      if (k > 0) r = mg_vcycle<T>(mg, 0, false, grun, h, false, mg->partial, &nparts, s);
+      if (k > 0) r = mg_vcycle<T>(mg, 0, false, grun, h, zg && k == 1, mg->partial, &nparts, s);
+      else if (zg) nparts = norm_blocks(mg->H, mg->W);
       if (!r && nparts < 0) {
         nparts = mg->npartial;
         r = residual_partials_dev<T>(reinterpret_cast<const T*>(L0.p[L0.cur]), div, L0.flags,
@@ -3256,6 +3281,12 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
     cycles = fin.cycles;
     rel = fin.rel;
     converged = rel <= prm->tolerance;
+    if (zg && cycles == 0) {  // no V-cycle ran: the result is the zero guess
+      hipLaunchKernelGGL(k_fill_gated<T>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, (T)0,
+                         nullptr);
+      rc = check_launch("k_fill_gated");
+      if (rc) return rc;
+    }
   } else if (prm->tolerance > 0.0) {
     rc = rhs_norm2_dev<T>(div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 1, s);
     if (rc) return rc;
@@ -3361,12 +3392,25 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
   const long n = (long)H * W;
   const int inner = prm->inner_cycles > 0 ? prm->inner_cycles : 1;
   const double tol = prm->tolerance;
-  rc = rhs_norm2_dev<double>(div, L0.flags, H, W, alpha, mg->partial, mg->result, 1, s);
+  // zero guess: in the gated loop the rhs norm's kernel also writes the
+  // defect of p = 0 and the first correction reads no p; the other loop
+  // starts from a zeroed p
+  const bool zg = prm->zero_guess != 0;
+  const bool gated = gateable(mg, run, 1);
+  float* const defect0 = reinterpret_cast<float*>(L0.rhs32);
+  rc = rhs_norm2_dev<double>(div, L0.flags, H, W, alpha, mg->partial, mg->result, 1, s,
+                             zg && gated ? defect0 : nullptr);
   if (rc) return rc;
+  if (zg && !gated) {
+    hipLaunchKernelGGL(k_fill_gated<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, 0.0,
+                       nullptr);
+    rc = check_launch("k_fill_gated");
+    if (rc) return rc;
+  }
   int refinements = 0, inner_total = 0;
   double rel = -1.0, b2 = 0.0;
   double host[2];
-  if (gateable(mg, run, 1)) {
+  if (gated) {
     // iteration 0: defect of the initial p + check; iteration k > 0: inner
     // FP32 V-cycles, p += e, defect + check -- all gated on the device state
     MgRun grun = run;
@@ -3389,15 +3433,16 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
         if (r) return r;
         const float* e = reinterpret_cast<const float*>(L0.p32[L0.cur32]);
         hipEvent_t ev = prof_begin(s);
+        const double* pin = (zg && k == 1) ? nullptr : buf[(k - 1) & 1];
         hipLaunchKernelGGL(k_correct_defect<double>, dim3(cdiv(W, NB_X), cdiv(H, NB_Y)),
-                           dim3(NB_T), 0, s, buf[(k - 1) & 1], e, buf[k & 1], div, L0.flags, H,
-                           W, alpha, mg->partial, defect, grun.gate);
+                           dim3(NB_T), 0, s, pin, e, buf[k & 1], div, L0.flags, H, W, alpha,
+                           mg->partial, defect, grun.gate);
         prof_end(ev, "correct_defect_fp64", 32.0 * (double)n, s);
         r = check_launch("k_correct_defect");
-      } else {
+      } else if (!zg) {
         r = residual_partials_dev<double>(p, div, L0.flags, H, W, alpha, mg->partial, defect,
                                           grun.gate, s);
-      }
+      }  // zero guess: the rhs norm's partials and defect stand for p = 0
       if (r) return r;
       hipLaunchKernelGGL(k_check, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
                          mg->result + 1, mg->d_state, tol, prm->v_cycles, inner, slot, sq);
@@ -3410,6 +3455,12 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     if (refinements & 1)
       SMK_CHECK_HIP(hipMemcpyAsync(p, buf[1], (size_t)n * sizeof(double),
                                    hipMemcpyDeviceToDevice, s));
+    if (zg && refinements == 0) {  // no refinement ran: the result is the zero guess
+      hipLaunchKernelGGL(k_fill_gated<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, 0.0,
+                         nullptr);
+      rc = check_launch("k_fill_gated");
+      if (rc) return rc;
+    }
   }
   while (!gateable(mg, run, 1)) {
     // defect in FP32 into level-0 rhs32, norm^2 of r into result[0]

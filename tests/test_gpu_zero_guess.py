@@ -1,0 +1,96 @@
+"""MultigridSolver(initial_guess="zero"): the caller's pressure.zero_()
+folded into the solve.  Whatever ``pressure`` holds on entry, the result and
+the solve info equal those of a solve from a zeroed ``pressure``, bit for bit
+(the initial residual of p = 0 is taken from the right-hand side's norm, the
+first V-cycle or refinement reads no p), in every loop the solver runs:
+tolerance-gated, fixed cycles, mixed precision, each smoother; and a solve
+that converges before any cycle still writes zeros."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(H, W, seed=3):
+    rng = np.random.default_rng(seed)
+    solid = np.zeros((H, W), dtype=bool)
+    solid[0, :] = solid[-1, :] = True
+    solid[:, 0] = solid[:, -1] = True
+    solid[H // 4:H // 2, W // 3:W // 2] = True
+    fmu = np.zeros((H, W + 1), dtype=bool)
+    fmv = np.zeros((H + 1, W), dtype=bool)
+    fmu[:, 1:-1] = ~(solid[:, :-1] | solid[:, 1:])
+    fmv[1:-1, :] = ~(solid[:-1, :] | solid[1:, :])
+    u = rng.standard_normal((H, W + 1))
+    v = rng.standard_normal((H + 1, W))
+    d = np.zeros((H, W))
+    O.compute_divergence(u, v, d, fmu, fmv, 1.0 / H)
+    return d, fmu, fmv
+
+
+def solve(H, W, dtype, guess, start, div_scale=1.0, **kw):
+    from cfd_amd.backends.hip import solvers
+
+    d, fmu, fmv = problem(H, W)
+    dev = "cuda"
+    out = torch.full((H, W), start, dtype=dtype, device=dev)
+    div = torch.from_numpy(d * div_scale).to(dev, dtype)
+    solvers.MultigridSolver()(out, div, torch.from_numpy(fmu).to(dev),
+                              torch.from_numpy(fmv).to(dev), 1.0, 1.0 / H, 0.005,
+                              initial_guess=guess, **kw)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), dict(solvers.LAST_SOLVE_INFO)
+
+
+CASES = {
+    "rb-tol-f64": (torch.float64, dict(smoother_type="rb_sor_gauss_seidel", smoother_omega=1.0,
+                                       tolerance=1e-8, v_cycles=40)),
+    "rb-tol-f32": (torch.float32, dict(smoother_type="rb_sor_gauss_seidel", smoother_omega=1.0,
+                                       tolerance=1e-5, v_cycles=40)),
+    "jacobi-tol-f64": (torch.float64, dict(smoother_type="jacobi", tolerance=1e-6, v_cycles=40)),
+    "rb-fixed-f64": (torch.float64, dict(smoother_type="rb_sor_gauss_seidel", v_cycles=3)),
+    "gs-tol-f64": (torch.float64, dict(smoother_type="gauss_seidel", tolerance=1e-6,
+                                       v_cycles=40)),
+    "mixed-f64": (torch.float64, dict(smoother_type="rb_sor_gauss_seidel", smoother_omega=1.0,
+                                      tolerance=1e-8, v_cycles=40, precision="mixed",
+                                      inner_cycles=2, coarse_scale="rediscretized")),
+    "mixed-fast-f64": (torch.float64, dict(smoother_type="rb_sor_gauss_seidel",
+                                           smoother_omega=1.0, tolerance=1e-8, v_cycles=40,
+                                           precision="mixed", inner_cycles=2,
+                                           coarsest_solve="operator",
+                                           inner_arithmetic="float32")),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("shape", [(128, 128, 4), (64, 96, 3)], ids=lambda s: "%dx%d" % s[:2])
+def test_zero_guess_equals_zeroed_pressure(case, shape):
+    H, W, levels = shape
+    dtype, kw = CASES[case]
+    if kw["smoother_type"] == "gauss_seidel" and H > 64:
+        pytest.skip("the lexicographic smoother is one workgroup per level: kept small")
+    ref, iref = solve(H, W, dtype, "given", 0.0, num_levels=levels, **kw)
+    got, info = solve(H, W, dtype, "zero", 123.25, num_levels=levels, **kw)
+    np.testing.assert_array_equal(got, ref)
+    assert info == iref
+    assert info["iterations"] > 0
+
+
+@pytest.mark.parametrize("case", ["rb-tol-f64", "mixed-f64"])
+def test_zero_rhs_writes_zeros(case):
+    """A zero right-hand side converges before any cycle: the result is the
+    zero guess, written over whatever pressure held."""
+    dtype, kw = CASES[case]
+    ref, iref = solve(64, 64, dtype, "given", 0.0, div_scale=0.0, num_levels=3, **kw)
+    got, info = solve(64, 64, dtype, "zero", -7.5, div_scale=0.0, num_levels=3, **kw)
+    assert info["iterations"] == iref["iterations"] == 0
+    np.testing.assert_array_equal(got, np.zeros((64, 64)))
+    np.testing.assert_array_equal(ref, np.zeros((64, 64)))
+
+
+def test_bad_initial_guess_is_rejected():
+    with pytest.raises(ValueError, match="initial_guess must be one of"):
+        solve(64, 64, torch.float64, "previous", 0.0, num_levels=3, v_cycles=1)
