@@ -122,7 +122,8 @@ TRAFFIC_FAMILIES = [
     ("k_residual_norm<double", "residual_norm_f64"), ("k_correct_defect<double", "correct_defect_fp64"),
     ("k_advect_velocity<double", "advect_velocity_f64"), ("k_advect_scalar<double", "advect_scalar_f64"),
     ("k_pressure_gradient<double", "pressure_gradient_f64"), ("k_divergence<double", "divergence_f64"),
-    ("k_residual_masks<double", "residual_f64"), ("k_apply_forces<double", "apply_forces_f64")]
+    ("k_residual_masks<double", "residual_f64"), ("k_apply_forces<double", "apply_forces_f64"),
+    ("k_div_residual<double", "div_residual_f64")]
 
 
 def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30) -> dict:

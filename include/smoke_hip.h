@@ -126,6 +126,15 @@ int smk_residual(const void* p, const void* div, const uint8_t* is_fluid,
                  const uint8_t* fmu, const uint8_t* fmv, int H, int W,
                  double alpha, double reciprocal_beta, void* r, int dtype,
                  void* stream);
+/* The step's metrics pass, simulation.py:251-265: smk_divergence of (u, v)
+ * into div, then smk_residual of p against that div into r, in one launch
+ * (each cell's residual reads only its own divergence); both outputs equal
+ * the two calls', bit for bit. */
+int smk_divergence_residual(const void* u, const void* v, const void* p,
+                            const uint8_t* is_fluid, const uint8_t* fmu,
+                            const uint8_t* fmv, int H, int W, double cell_size,
+                            double alpha, double reciprocal_beta, void* div,
+                            void* r, int dtype, void* stream);
 
 /* ------------------------------------------------------------- emitters */
 

@@ -129,6 +129,8 @@ SIGNATURES = {
     "smk_pressure_gradient": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _vp]),
     "smk_enforce_boundaries": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
     "smk_residual": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i, _vp]),
+    "smk_divergence_residual": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _vp, _vp,
+                                     _i, _vp]),
     "smk_emit_scalar": (_i, [_vp, _i, _i, _d, _d, _d, _d, _d, _i, _i, _vp]),
     "smk_box_source": (_i, [_vp, _i, _i, _i, _i, _i, _i, _d, _i, _i, _vp]),
     "smk_solver_workspace_bytes": (_sz, [_i, _i, _i]),

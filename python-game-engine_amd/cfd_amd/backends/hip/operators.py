@@ -236,6 +236,33 @@ def compute_residual(pressure, divergence, is_fluid, face_mask_u, face_mask_v, a
     return residual_out
 
 
+def divergence_residual(velocity_u, velocity_v, pressure, is_fluid, face_mask_u, face_mask_v,
+                        cell_size, alpha, reciprocal_beta, divergence, residual_out):
+    """The step's metrics pass (simulation.py:251-265) in one launch:
+    compute_divergence(velocity_u, velocity_v, divergence, ...) then
+    compute_residual(pressure, divergence, ...) into residual_out, with the
+    same results as the two calls."""
+    _field(pressure, "pressure")
+    dt_ = pressure.dtype
+    H, W = pressure.shape
+    for t, name, shape in ((velocity_u, "velocity_u", (H, W + 1)),
+                           (velocity_v, "velocity_v", (H + 1, W)),
+                           (divergence, "divergence", (H, W)),
+                           (residual_out, "residual_out", (H, W))):
+        _field(t, name, dt_)
+        _shape(t, shape, name)
+    _mask(is_fluid, "is_fluid", (H, W))
+    _mask(face_mask_u, "face_mask_u", (H, W + 1))
+    _mask(face_mask_v, "face_mask_v", (H + 1, W))
+    if reciprocal_beta == 0:
+        raise ValueError("reciprocal_beta must be non-zero")
+    call("smk_divergence_residual", _p(velocity_u), _p(velocity_v), _p(pressure), _p(is_fluid),
+         _p(face_mask_u), _p(face_mask_v), H, W, float(cell_size), float(alpha),
+         float(reciprocal_beta), _p(divergence), _p(residual_out), dtype_code(pressure),
+         stream_of(pressure))
+    return residual_out
+
+
 # ------------------------------------------------------------------ masks
 def face_masks_from_solid(solid_mask, face_mask_u, face_mask_v):
     """simulation.py:200-202 / multigrid/masks.py:30-49 (outer faces closed)."""

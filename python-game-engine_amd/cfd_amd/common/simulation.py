@@ -218,14 +218,22 @@ class SmokeSimulation2D:
 
         if self.compute_metrics:
             with t.scope("profiling.metrics"):
-                self._compute_divergence()
-                prof.metrics.capture("divergence", "after_projection", self.divergence)
-                prof.metrics.capture("pressure", "final", self.pressure)
                 alpha = calculate_alpha(self.cell_size, self.fluid_density, dt)
-                compute_residual(self.pressure, self.divergence, self.is_fluid, self._face_mask_u,
-                                 self._face_mask_v, alpha, calculate_reciprocal_beta(),
-                                 self.pressure_residual,
-                                 backend=self.operators_backend.residual)
+                if fused:  # divergence and residual in one launch, same results
+                    hip.divergence_residual(self.velocity_u, self.velocity_v, self.pressure,
+                                            self.is_fluid, self._face_mask_u, self._face_mask_v,
+                                            self.cell_size, alpha, calculate_reciprocal_beta(),
+                                            self.divergence, self.pressure_residual)
+                    prof.metrics.capture("divergence", "after_projection", self.divergence)
+                    prof.metrics.capture("pressure", "final", self.pressure)
+                else:
+                    self._compute_divergence()
+                    prof.metrics.capture("divergence", "after_projection", self.divergence)
+                    prof.metrics.capture("pressure", "final", self.pressure)
+                    compute_residual(self.pressure, self.divergence, self.is_fluid,
+                                     self._face_mask_u, self._face_mask_v, alpha,
+                                     calculate_reciprocal_beta(), self.pressure_residual,
+                                     backend=self.operators_backend.residual)
                 prof.metrics.capture("pressure_residual", "final", self.pressure_residual)
                 prof.metrics.process()
 

@@ -348,6 +348,42 @@ __global__ void k_residual_masks(const T* __restrict__ p, const T* __restrict__ 
   r[o] = (T)(((double)nsum + alpha * (double)div[o]) - beta * (double)c);
 }
 
+// The step's metrics pass (simulation.py:251-265): compute_divergence of the
+// projected velocity, then compute_residual of p against that divergence, in
+// one launch.  A cell's residual needs only its own divergence, so each
+// thread forms div exactly as k_divergence does, stores it and uses the
+// stored value exactly as k_residual_masks does: both outputs equal the two
+// separate launches', bit for bit.
+template <typename T>
+__global__ void k_div_residual(const T* __restrict__ u, const T* __restrict__ v,
+                               const T* __restrict__ p, const uint8_t* __restrict__ fluid,
+                               const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv,
+                               int H, int W, double inv, double alpha, double beta,
+                               T* __restrict__ div, T* __restrict__ r) {
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
+  if (j >= H || i >= W) return;
+  const size_t a = (size_t)j * (W + 1) + i;
+  const size_t o = (size_t)j * W + i;
+  const uint8_t mu0 = fmu[a], mu1 = fmu[a + 1], mv0 = fmv[o], mv1 = fmv[o + W];
+  const T ddu = (u[a + 1] * (T)mu1) - (u[a] * (T)mu0);
+  const T ddv = (v[o + W] * (T)mv1) - (v[o] * (T)mv0);
+  const T d = (T)((double)ddu * inv + (double)ddv * inv);
+  div[o] = d;
+  if (!fluid[o]) {
+    r[o] = (T)0;
+    return;
+  }
+  const T c = p[o];
+  const T left = mu0 ? p[(size_t)j * W + (i - 1 > 0 ? i - 1 : 0)] : c;
+  const T right = mu1 ? p[(size_t)j * W + (i + 1 < W - 1 ? i + 1 : W - 1)] : c;
+  const T down = mv0 ? p[(size_t)(j - 1 > 0 ? j - 1 : 0) * W + i] : c;
+  const T up = mv1 ? p[(size_t)(j + 1 < H - 1 ? j + 1 : H - 1) * W + i] : c;
+  const T nsum = left + right + down + up;
+  r[o] = (T)(((double)nsum + alpha * (double)d) - beta * (double)c);
+}
+
 // ------------------------------------------------------------- emitters
 // emitters.py:339-411 falloffs.
 __device__ __forceinline__ double falloff(double d, double r, int approach) {
@@ -794,6 +830,28 @@ int smk_divergence(const void* u, const void* v, void* div, const uint8_t* fmu,
                                   (const float*)v, (float*)div, fmu, fmv, H, W,
                                   1.0 / cell_size));
   return check_launch("k_divergence");
+}
+
+int smk_divergence_residual(const void* u, const void* v, const void* p,
+                            const uint8_t* is_fluid, const uint8_t* fmu, const uint8_t* fmv,
+                            int H, int W, double cell_size, double alpha,
+                            double reciprocal_beta, void* div, void* r, int dtype,
+                            void* stream) {
+  SMK_REQUIRE(u && v && p && is_fluid && fmu && fmv && div && r,
+              "smk_divergence_residual: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0 && cell_size != 0.0 && reciprocal_beta != 0.0,
+              "smk_divergence_residual: bad arguments");
+  const double inv = 1.0 / cell_size, beta = 1.0 / reciprocal_beta;
+  auto g = grid2d(H, W);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_div_residual<double>, g, dim3(BX, BY), 0, s,
+                                  (const double*)u, (const double*)v, (const double*)p, is_fluid,
+                                  fmu, fmv, H, W, inv, alpha, beta, (double*)div, (double*)r),
+               hipLaunchKernelGGL(k_div_residual<float>, g, dim3(BX, BY), 0, s,
+                                  (const float*)u, (const float*)v, (const float*)p, is_fluid,
+                                  fmu, fmv, H, W, inv, alpha, beta, (float*)div, (float*)r));
+  return check_launch("k_div_residual");
 }
 
 int smk_pressure_gradient(void* u, void* v, const void* p, const uint8_t* fmu,
