@@ -2,10 +2,13 @@
 // multigrid (reference scaling or rediscretised), FP32-in-FP64 defect
 // correction.
 //
-// Core kernel: k_smooth_tiled.  A workgroup owns a TX x TY tile of the
-// output, stages the tile plus an R-cell halo of p, rhs and the packed cell
-// flags in LDS once, then runs `npass` colour passes (RB) or iterations
-// (Jacobi) entirely in LDS.  After pass k only cells at distance >= k+1 from
+// Core kernels: the temporally blocked tile smoothers (k_smooth_tile64 and
+// its variants: k_smooth_fine / _r / _d for the finest level, _r / _d for
+// the others; the variants differ in region height, workgroup size and where
+// p and the rhs live while the passes run).  A workgroup owns a TX x TY tile
+// of the output, stages the tile plus an R-cell halo of p, rhs and the
+// packed cell flags once, then runs `npass` colour passes (RB) or iterations
+// (Jacobi) on chip.  After pass k only cells at distance >= k+1 from
 // the staged region's edge are exact, so R >= npass keeps the tile interior
 // exact: npass sweeps cost one HBM read and one write of p instead of npass.
 // The kernel reads p_in and writes p_out (ping-pong), so neighbouring tiles
