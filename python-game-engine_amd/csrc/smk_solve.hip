@@ -28,7 +28,6 @@
 // Python-float coefficients as float64).
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -2080,6 +2079,7 @@ __global__ void k_solid_from_flags(const uint8_t* __restrict__ flags, long n,
 constexpr int TILE_NT = 256;
 constexpr int TILE_RH = 48;
 constexpr int MAX_R = 8;
+constexpr int MAX_R_DEEP = 24;  // red-black single-grid launches on small grids
 constexpr int SMALL_NT = 256;
 
 inline int tile_tx(int R) { return 64 - 2 * R; }
@@ -2213,7 +2213,8 @@ int launch_tiled_once(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine
 
 template <typename T>
 int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = false) {
-  if (a.R < 1 || a.R > MAX_R + 1) return fail(SMK_EINVAL, "halo %d outside [1, %d]", a.R, MAX_R + 1);
+  const int rlim = mode == MODE_RB ? MAX_R_DEEP : MAX_R + 1;
+  if (a.R < 1 || a.R > rlim) return fail(SMK_EINVAL, "halo %d outside [1, %d]", a.R, rlim);
   hipEvent_t ev = prof_begin(s);
   const bool f32 = sizeof(T) == 4;
   const char* tag = mode == MODE_RB
@@ -2457,10 +2458,31 @@ inline SolverWs carve_ws(void* ws, int H, int W, int dtype) {
   return w;
 }
 
+// Passes per launch of the single-grid red-black solvers on grids of at
+// most RB_DEEP_CELLS cells, where a launch is latency-bound: a deeper halo
+// costs redundant passes on chip but saves launches (512^2 FP64 RB-SOR with
+// a check every 20 sweeps: 90.6 ms per step at 8 passes per launch, 74.8 at
+// 16, 68.5 at 20, with the 40 passes between checks in two launches).
+// Capped so that the finest-level variant keeps at least 16 exact rows per
+// region; only the register-rhs bodies take halos beyond MAX_R + 1.
+constexpr long RB_DEEP_CELLS = 1024L * 1024L;
+
+template <typename T>
+int max_passes_per_launch(int mode, int H, int W) {
+  if (mode != MODE_RB || (long)H * W > RB_DEEP_CELLS) return MAX_R;
+  const int v = g_variant[sizeof(T) == 4][0];
+  if (v != VAR_LDS64R && v != VAR_LDS48R && v != VAR_LDS32R) return MAX_R;
+  const int rh = variant_rh(v);
+  int c = MAX_R_DEEP;
+  while (c > MAX_R && rh - 2 * c < 16) --c;
+  return c;
+}
+
 // Runs `npass_total` passes from src into the buffer `dst_final`, using
 // `other` as the ping-pong partner; chunked into launches of at most
-// MAX_R passes.  The chunking keeps the launch count's parity such that the
-// last launch writes dst_final (src may equal dst_final).
+// max_passes_per_launch passes.  The chunking keeps the launch count's
+// parity such that the last launch writes dst_final (src may equal
+// dst_final).
 template <typename T>
 int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
                const uint8_t* flags, int H, int W, double alpha, double omega,
@@ -2472,12 +2494,11 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
                                    hipMemcpyDeviceToDevice, s));
     return SMK_OK;
   }
+  // as few launches as the cap allows, the passes spread evenly over them
   std::vector<int> chunks;
-  for (int left = npass_total; left > 0;) {
-    const int c = left < MAX_R ? left : MAX_R;
-    chunks.push_back(c);
-    left -= c;
-  }
+  const int cmax = max_passes_per_launch<T>(mode, H, W);
+  const int nl = (npass_total + cmax - 1) / cmax;
+  for (int q = 0; q < nl; ++q) chunks.push_back(npass_total / nl + (q < npass_total % nl ? 1 : 0));
   // launches alternate buffers: src -> X -> Y -> ... ; the last must be dst_final.
   // With src == dst_final an even launch count ends in dst_final; with src ==
   // other an odd count does.  Split a chunk to fix the parity, or copy.

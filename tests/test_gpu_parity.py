@@ -212,7 +212,9 @@ def test_jacobi_bitwise(hip, shape, iterations):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("iterations,omega", [(1, 1.0), (3, 1.7), (10, 0.0), (9, 1.9)])
+# 9, 10 and 24 sweeps run as one or two launches of 18, 20 and 24 passes
+# (halos deeper than the multigrid's) on these small grids
+@pytest.mark.parametrize("iterations,omega", [(1, 1.0), (3, 1.7), (10, 0.0), (9, 1.9), (24, 1.3)])
 def test_rbsor_bitwise(hip, shape, iterations, omega):
     _, sol = hip
     H, W = shape
