@@ -2213,7 +2213,7 @@ int launch_tiled_once(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine
 
 template <typename T>
 int launch_tiled(int mode, const SmoothArgs<T>& a, hipStream_t s, bool fine = false) {
-  const int rlim = mode == MODE_RB ? MAX_R_DEEP : MAX_R + 1;
+  const int rlim = mode == MODE_RB ? MAX_R_DEEP : (JAC_RH - 8) / 2;
   if (a.R < 1 || a.R > rlim) return fail(SMK_EINVAL, "halo %d outside [1, %d]", a.R, rlim);
   hipEvent_t ev = prof_begin(s);
   const bool f32 = sizeof(T) == 4;
@@ -2458,8 +2458,8 @@ inline SolverWs carve_ws(void* ws, int H, int W, int dtype) {
   return w;
 }
 
-// Passes per launch of the single-grid red-black solvers on grids of at
-// most RB_DEEP_CELLS cells, where a launch is latency-bound: a deeper halo
+// Passes per launch of the single-grid solvers (red-black and Jacobi) on
+// grids of at most RB_DEEP_CELLS cells, where a launch is latency-bound: a deeper halo
 // costs redundant passes on chip but saves launches (512^2 FP64 RB-SOR with
 // a check every 20 sweeps: 90.6 ms per step at 8 passes per launch, 74.8 at
 // 16, 68.5 at 20, with the 40 passes between checks in two launches).
@@ -2469,6 +2469,9 @@ constexpr long RB_DEEP_CELLS = 1024L * 1024L;
 
 template <typename T>
 int max_passes_per_launch(int mode, int H, int W) {
+  // Jacobi: the 32-row regions keep at least 8 exact rows (12 passes; the
+  // 128^2 Jacobi-200 workload 0.318 -> 0.273 ms per step against 8)
+  if (mode == MODE_JACOBI && (long)H * W <= RB_DEEP_CELLS) return (JAC_RH - 8) / 2;
   if (mode != MODE_RB || (long)H * W > RB_DEEP_CELLS) return MAX_R;
   const int v = g_variant[sizeof(T) == 4][0];
   if (v != VAR_LDS64R && v != VAR_LDS48R && v != VAR_LDS32R) return MAX_R;
