@@ -260,6 +260,17 @@ typedef struct smk_mg_params {
  * are even, up to num_levels) and allocates every level buffer. */
 int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W,
                   int num_levels, int dtype, void* stream, smk_mg** out);
+/* multigrid/solver.py:34-47, :87-92: the mask side of the reference's level
+ * cache, in place and without allocation.  all_levels != 0: every level's
+ * masks are derived again from fmu/fmv (what _build_levels does on a rebuild,
+ * rebuild_masks=True or a level count short of num_levels).  all_levels == 0:
+ * level 0 follows face-mask arrays that changed in place -- the reference
+ * keeps level 0's face-mask arrays by reference, so its level-0 smoother
+ * sees their current content, while level 0's is_fluid (read by the
+ * residual, restriction and prolongation) and the coarse levels stay as
+ * built.  Stream-ordered on `stream`. */
+int smk_mg_refresh_masks(smk_mg* mg, const uint8_t* fmu, const uint8_t* fmv,
+                         int all_levels, void* stream);
 /* Number of levels actually built. */
 int smk_mg_num_levels(const smk_mg* mg);
 /* multigrid/solver.py:141-176 MultigridSolver.__call__ (solver key

@@ -57,6 +57,8 @@ _SIGS = {
     "orc_restrict": (None, [_dp, _u8, _i, _i, _dp]),
     "orc_prolong": (None, [_dp, _i, _i, _u8, _i, _i, _dp]),
     "orc_mg_solve": (_i, [_dp, _dp, _u8, _u8, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i, _d, _i, _d, _dp]),
+    "orc_mg_solve_cached": (_i, [_dp, _dp, _u8, _u8, _u8, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i,
+                                 _d, _i, _d, _dp]),
     "orc_falloff": (_d, [_d, _d, _i]),
     "orc_emit_scalar": (None, [_dp, _i, _i, _d, _d, _d, _d, _d, _i]),
     "orc_num_threads": (_i, []),
@@ -309,21 +311,31 @@ def multigrid_solver(p, div, face_mask_u, face_mask_v, fluid_density, cell_size,
                      dt, v_cycles=2, smoother_type="jacobi", num_levels=4,
                      smoother_iterations=2, coarsest_iterations=50,
                      smoother_omega=0.0, coarse_scale="reference", tolerance=0.0,
-                     **_ignored):
+                     level_is_fluid=None, **_ignored):
     """multigrid/solver.py:141-176 (fresh cache per call).
 
     ``coarse_scale="reference"`` keeps solver.py:124-125 (restrict(r)/alpha(2h));
     ``"rediscretized"`` divides by alpha(h) instead (new option).
 
+    ``level_is_fluid``: level 0's is_fluid as an earlier call's cache built
+    it (solver.py:87-92 without a rebuild), while the face masks passed in
+    are the arrays that cache holds, changed in place since: the smoothers
+    see the masks, the level-0 residual, restriction and prolongation and
+    the coarse levels the cached is_fluid.  None: a fresh cache.
+
     Returns (v-cycles run, relative residual or -1 when no tolerance)."""
     H, W = p.shape
     rel = ctypes.c_double(0.0)
-    n = lib().orc_mg_solve(_f(p), _f(div), _m(_mask(face_mask_u)), _m(_mask(face_mask_v)),
-                           H, W, fluid_density, cell_size, dt, int(v_cycles),
-                           SMOOTHERS[smoother_type], int(num_levels),
-                           int(smoother_iterations), int(coarsest_iterations),
-                           float(smoother_omega), 1 if coarse_scale == "rediscretized" else 0,
-                           float(tolerance), ctypes.byref(rel))
+    lf = None if level_is_fluid is None else _mask(level_is_fluid)
+    if lf is not None and lf.shape != (H, W):
+        raise ValueError("level_is_fluid must have the shape of p")
+    n = lib().orc_mg_solve_cached(_f(p), _f(div), _m(_mask(face_mask_u)), _m(_mask(face_mask_v)),
+                                  None if lf is None else _m(lf), H, W, fluid_density,
+                                  cell_size, dt, int(v_cycles), SMOOTHERS[smoother_type],
+                                  int(num_levels), int(smoother_iterations),
+                                  int(coarsest_iterations), float(smoother_omega),
+                                  1 if coarse_scale == "rediscretized" else 0, float(tolerance),
+                                  ctypes.byref(rel))
     return n, rel.value
 
 

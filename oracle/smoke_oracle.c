@@ -576,6 +576,28 @@ int orc_mg_solve(double* p, const double* div, const uint8_t* fmu,
                  int num_levels, int smoother_iterations,
                  int coarsest_iterations, double smoother_omega,
                  int rediscretize, double tol, double* rel_out) {
+  return orc_mg_solve_cached(p, div, fmu, fmv, NULL, H, W, rho, cell_size, dt,
+                             v_cycles, smoother, num_levels, smoother_iterations,
+                             coarsest_iterations, smoother_omega, rediscretize,
+                             tol, rel_out);
+}
+
+/* As orc_mg_solve, with the level cache of solver.py:87-92 left in place
+ * after the face-mask arrays changed: level_fluid (H x W, may be NULL =
+ * derived from fmu/fmv) is level 0's is_fluid as the cache built it.  The
+ * level-0 face masks are fmu/fmv (the cache holds the arrays, solver.py:45-46)
+ * and each smoother call derives its own is_fluid from them
+ * (red_black_sor_gauss_seidel.py:38, jacobi.py:46, gauss_seidel.py:46);
+ * the residual, restriction and prolongation of level 0 read level_fluid
+ * (solver.py:98,116,122,133), and the coarse levels are built from
+ * solid = ~level_fluid (solver.py:40). */
+int orc_mg_solve_cached(double* p, const double* div, const uint8_t* fmu,
+                        const uint8_t* fmv, const uint8_t* level_fluid, int H,
+                        int W, double rho, double cell_size, double dt,
+                        int v_cycles, int smoother, int num_levels,
+                        int smoother_iterations, int coarsest_iterations,
+                        double smoother_omega, int rediscretize, double tol,
+                        double* rel_out) {
   if (num_levels < 1) num_levels = 1;
   orc_level* lv = (orc_level*)calloc((size_t)num_levels, sizeof(orc_level));
   /* solver.py:34-73 _build_levels */
@@ -586,7 +608,10 @@ int orc_mg_solve(double* p, const double* div, const uint8_t* fmu,
     L->fmu = (uint8_t*)fmu; L->fmv = (uint8_t*)fmv; L->owns_masks = 0;
     L->fl = (uint8_t*)malloc((size_t)H * W);
     L->solid = (uint8_t*)malloc((size_t)H * W);
-    orc_is_fluid(fmu, fmv, H, W, L->fl);
+    if (level_fluid)
+      memcpy(L->fl, level_fluid, (size_t)H * W);
+    else
+      orc_is_fluid(fmu, fmv, H, W, L->fl);
     for (size_t k = 0; k < (size_t)H * W; ++k) L->solid[k] = !L->fl[k];
     nl = 1;
   }

@@ -144,6 +144,13 @@ int orc_mg_solve(double* p, const double* div, const uint8_t* fmu,
                  int num_levels, int smoother_iterations,
                  int coarsest_iterations, double smoother_omega,
                  int rediscretize, double tol, double* rel_out);
+int orc_mg_solve_cached(double* p, const double* div, const uint8_t* fmu,
+                        const uint8_t* fmv, const uint8_t* level_fluid, int H,
+                        int W, double rho, double cell_size, double dt,
+                        int v_cycles, int smoother, int num_levels,
+                        int smoother_iterations, int coarsest_iterations,
+                        double smoother_omega, int rediscretize, double tol,
+                        double* rel_out);
 
 /* emitters.py:6-35 falloff, approach 0 hard_cutoff, 1 linear, 2 quadratic,
  * 3 gaussian */

@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "lib", "libsmoke_hip.so"))
 
-ABI_VERSION = 4  # smk_abi_version() of the library these bindings describe
+ABI_VERSION = 5  # smk_abi_version() of the library these bindings describe
 
 SMK_F64 = 0
 SMK_F32 = 1
@@ -143,6 +143,7 @@ SIGNATURES = {
     "smk_residual_norm2": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _sz,
                                 ctypes.POINTER(ctypes.c_double), _i, _vp]),
     "smk_mg_create": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, ctypes.POINTER(_vp)]),
+    "smk_mg_refresh_masks": (_i, [_vp, _vp, _vp, _i, _vp]),
     "smk_mg_num_levels": (_i, [_vp]),
     "smk_mg_solve": (_i, [_vp, _vp, _vp, _d, _d, _d, ctypes.POINTER(MgParams),
                           ctypes.POINTER(SolveInfo), _vp]),

@@ -264,6 +264,14 @@ def divergence_residual(velocity_u, velocity_v, pressure, is_fluid, face_mask_u,
 
 
 # ------------------------------------------------------------------ masks
+def _written(*tensors):
+    """Bump the version counters of mask tensors a kernel wrote in place
+    (writes through a raw pointer do not), so that holders of those tensors
+    -- the multigrid solver's level-0 cache -- see the change."""
+    for t in tensors:
+        torch.autograd.graph.increment_version(t)
+
+
 def face_masks_from_solid(solid_mask, face_mask_u, face_mask_v):
     """simulation.py:200-202 / multigrid/masks.py:30-49 (outer faces closed)."""
     _mask(solid_mask, "solid_mask")
@@ -272,6 +280,7 @@ def face_masks_from_solid(solid_mask, face_mask_u, face_mask_v):
     _mask(face_mask_v, "face_mask_v", (H + 1, W))
     call("smk_face_masks_from_solid", _p(solid_mask), H, W, _p(face_mask_u), _p(face_mask_v),
          stream_of(solid_mask))
+    _written(face_mask_u, face_mask_v)
 
 
 def is_fluid_from_masks(face_mask_u, face_mask_v, out):
@@ -281,6 +290,7 @@ def is_fluid_from_masks(face_mask_u, face_mask_v, out):
     _mask(face_mask_v, "face_mask_v", (H + 1, W))
     _mask(out, "is_fluid", (H, W))
     call("smk_is_fluid", _p(face_mask_u), _p(face_mask_v), H, W, _p(out), stream_of(out))
+    _written(out)
     return out
 
 
@@ -290,6 +300,7 @@ def coarsen_solid_mask(solid_fine, solid_coarse_out):
     H, W = solid_fine.shape
     _mask(solid_coarse_out, "solid_coarse_out", (H // 2, W // 2))
     call("smk_coarsen_solid", _p(solid_fine), H, W, _p(solid_coarse_out), stream_of(solid_fine))
+    _written(solid_coarse_out)
     return solid_coarse_out
 
 

@@ -160,10 +160,18 @@ class _MgHandle:
 class MultigridSolver:
     """multigrid/solver.py:16-177 on the GPU.
 
-    Level masks and buffers are cached per instance and rebuilt only when the
-    grid shape, level count, dtype or device changes or ``rebuild_masks`` is
-    passed -- the reference's cache rule (solver.py:87-92), which also means a
-    change of mask *content* under the same shape needs ``rebuild_masks=True``.
+    Level masks and buffers are cached per instance under the reference's
+    cache rule (solver.py:87-92): the masks of every level are derived again
+    from the face masks passed in when ``rebuild_masks`` is set or when the
+    hierarchy has fewer levels than ``num_levels`` (the reference then
+    rebuilds on every call); otherwise they stay as built, except that level
+    0 keeps the face-mask *tensors* of the build, as the reference keeps the
+    arrays (solver.py:42-47): when their content changes in place (seen
+    through torch's version counter; the simulation's mask kernels bump it),
+    the level-0 smoother follows them while the level-0 residual,
+    restriction and prolongation keep the is_fluid of the build, as the
+    reference's do.  Buffers are reallocated only when the grid shape, level
+    count, dtype or device changes; mask refreshes are in place.
 
     Smoothing is fused into the V-cycle kernels, so ``smoother_backend`` is
     accepted for signature compatibility and the smoother always runs on the
@@ -182,19 +190,45 @@ class MultigridSolver:
     def __init__(self):
         self._key = None
         self._handle = None
+        self._level0 = None    # (face_mask_u, face_mask_v) tensors level 0 holds
+        self._versions = None  # their versions when the masks were last derived
+        self._fresh = False    # every level derived from them (no stale level-0 is_fluid)
 
     @property
     def num_levels_built(self) -> int:
         return self._handle.levels if self._handle is not None else 0
 
+    def _remember(self, face_mask_u, face_mask_v, fresh):
+        self._level0 = (face_mask_u, face_mask_v)
+        self._versions = (face_mask_u._version, face_mask_v._version)
+        self._fresh = fresh
+
     def _ensure(self, face_mask_u, face_mask_v, num_levels, pressure, rebuild):
         key = (tuple(face_mask_u.shape), int(num_levels), pressure.dtype, pressure.device)
-        if rebuild or self._handle is None or self._key != key:
+        if self._handle is None or self._key != key:
             if self._handle is not None:
                 torch.cuda.current_stream(pressure.device).synchronize()
                 self._handle.close()
             self._handle = _MgHandle(face_mask_u, face_mask_v, num_levels, dtype_code(pressure))
             self._key = key
+            self._remember(face_mask_u, face_mask_v, True)
+            return
+        if rebuild or self._handle.levels != int(num_levels):
+            # solver.py:87-90 rebuilds from the masks passed in; skipped when
+            # that would reproduce the hierarchy already held
+            f0, v0 = self._level0
+            same = (f0 is face_mask_u and v0 is face_mask_v
+                    and self._versions == (face_mask_u._version, face_mask_v._version))
+            if not (same and self._fresh):
+                call("smk_mg_refresh_masks", self._handle.ptr, face_mask_u.data_ptr(),
+                     face_mask_v.data_ptr(), 1, stream_of(pressure))
+                self._remember(face_mask_u, face_mask_v, True)
+            return
+        f0, v0 = self._level0
+        if self._versions != (f0._version, v0._version):
+            call("smk_mg_refresh_masks", self._handle.ptr, f0.data_ptr(), v0.data_ptr(), 0,
+                 stream_of(pressure))
+            self._remember(f0, v0, False)
 
     def __call__(self, pressure, divergence, face_mask_u, face_mask_v, fluid_density, cell_size,
                  dt, v_cycles=2, smoother_type="jacobi", smoother_backend="hip", num_levels=4,

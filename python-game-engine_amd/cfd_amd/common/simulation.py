@@ -10,8 +10,14 @@ torch tensors on the GPU in ``config.dtype``.  Differences, all deliberate:
   folded into the density advection, boundary enforcement folded into the
   pressure-gradient kernel;
 * temperature advection and cooling are skipped while the temperature field
-  is identically zero and the ambient temperature is zero (both then map
-  zero to zero exactly); ``track_temperature = True`` forces them.
+  is known to be identically zero and the ambient temperature is zero (both
+  then map zero to zero exactly).  The field stops being known zero, for the
+  rest of the run, once the ambient temperature is non-zero, an emitter
+  writes temperature (an emitter without a ``writes_temperature`` attribute
+  is taken to write it), or the tensor is written through torch or replaced
+  (its version counter or identity changes).  Writes that bypass torch (a
+  raw device pointer) are not seen: set ``track_temperature = True`` for
+  those; it forces the temperature passes on.
 
 Reference quirks kept for parity: face masks are built before the boundary
 ring is marked solid and the mask is not flagged dirty (simulation.py:102-113),
@@ -118,6 +124,7 @@ class SmokeSimulation2D:
 
         self.track_temperature = None  # None: automatic (see module docstring)
         self._temperature_live = False
+        self._watch_temperature()
         self.compute_metrics = True
         self.last_solve_info: dict = {}
 
@@ -134,7 +141,10 @@ class SmokeSimulation2D:
             raise ValueError(f"Solid mask must have shape ({self.height}, {self.width})")
         if isinstance(solid_mask, np.ndarray):
             solid_mask = torch.from_numpy(solid_mask.astype(bool))
-        self._solid_mask = solid_mask.to(device=self.device, dtype=torch.bool).contiguous()
+        # a copy, as the reference's astype(bool): later edits of the caller's
+        # tensor do not reach the simulation unless set again
+        self._solid_mask = solid_mask.to(device=self.device, dtype=torch.bool).clone(
+            memory_format=torch.contiguous_format)
         self._mask_dirty = True
 
     def get_solid_mask(self):
@@ -155,11 +165,20 @@ class SmokeSimulation2D:
         ob = self.operators_backend
         return all(Backend(getattr(ob, k)) == Backend.HIP for k in vars(ob))
 
+    def _watch_temperature(self):
+        """Remember the temperature tensor and its version while it is known
+        to be zero."""
+        self._temperature_ref = self.temperature
+        self._temperature_version = self.temperature._version
+
     def _temperature_active(self) -> bool:
         if self.track_temperature is not None:
             return bool(self.track_temperature)
-        if self.ambient_temperature != 0.0 or any(
-                getattr(e, "writes_temperature", False) for e in self.emitters):
+        if not self._temperature_live and (
+                self.ambient_temperature != 0.0
+                or any(getattr(e, "writes_temperature", True) for e in self.emitters)
+                or self.temperature is not self._temperature_ref
+                or self.temperature._version != self._temperature_version):
             self._temperature_live = True
         return self._temperature_live
 
@@ -239,8 +258,7 @@ class SmokeSimulation2D:
 
     # --------------------------------------------------------- components
     def _apply_emitters(self, dt):
-        temp = self.temperature if self._temperature_active() or any(
-            getattr(e, "writes_temperature", False) for e in self.emitters) else None
+        temp = self.temperature if self._temperature_active() else None
         for e in self.emitters:
             e.emit(self.density, self.velocity_u, self.velocity_v, temp, dt)
 
@@ -330,4 +348,5 @@ class SmokeSimulation2D:
             f.zero_()
         self.accumulator = 0.0
         self._temperature_live = False
+        self._watch_temperature()
 

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(NB_T) void k_b_defect(const BatchScene* __restrict_
       const uint8_t f = d.flags0[o];
       const double* p = d.p;
       double rv = 0.0;
-      if (f & F_FLUID) {
+      if (f & F_LFLUID) {
         const double ce = p[o];
         const double l = (f & F_LEFT) ? p[o - 1] : ce;
         const double rr = (f & F_RIGHT) ? p[o + 1] : ce;
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(NB_T) void k_b_defect(const BatchScene* __restrict_
         rv = poisson_residual(l, rr, dn, up, ce, d.div[o], d.alpha0);
         acc = rv * rv;
       }
-      d.defect[o] = (f & F_FLUID) ? (float)(rv / d.alpha0) : 0.0f;
+      d.defect[o] = (f & F_LFLUID) ? (float)(rv / d.alpha0) : 0.0f;
     }
     const double tot = block_sum<NB_T>(acc, red);
     if (threadIdx.x == 0) partial[d.part_base + jb.z * d.nbx + jb.y] = tot;

@@ -49,18 +49,36 @@ inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 // the reference would read a *different* cell: an open face whose clamped
 // neighbour index equals the cell itself reads the cell's own value, exactly
 // like a closed face.
+//
+// Two fluid bits: F_FLUID is the is_fluid a smoother derives from the face
+// masks it is handed (jacobi.py:46, red_black_sor_gauss_seidel.py:38),
+// F_LFLUID the is_fluid a multigrid level holds (solver.py:38-47), which the
+// residual, the restriction, the prolongation and the residual norms read.
+// Both are set alike except on a cached multigrid level 0 whose face-mask
+// arrays changed in place since the hierarchy was built: the reference
+// keeps those arrays by reference and is_fluid by value, so its level-0
+// smoother sees the new masks and its level-0 residual the old is_fluid.
 enum : uint8_t {
   F_FLUID = 1u,
   F_LEFT = 2u,
   F_RIGHT = 4u,
   F_DOWN = 8u,
   F_UP = 16u,
+  F_LFLUID = 32u,
 };
+
+// Re-derive F_FLUID and the neighbour bits of flags from the face masks,
+// keeping each cell's F_LFLUID (stream-ordered, no allocation).
+int refresh_cell_flags(const uint8_t* fmu, const uint8_t* fmv, int H, int W, uint8_t* flags,
+                       hipStream_t s);
 
 // --------------------------------------------------------- sampling
 // advection.py:6-49: floor/ceil, clamp to [0, n-1], t = x - x0 (clamped x0),
 // lerp(a, b, t) = a + (b - a) * t.  Coordinates are float64 whatever the
-// field type (the reference's coordinates are Python floats).
+// field type (the reference's coordinates are Python floats).  The first two
+// lerps take two field values: (b - a) is formed in the field's type and
+// only then promoted by the float64 weight (Numba's typing of
+// lerp(float32, float32, float64)); the last lerp is float64 throughout.
 __device__ __forceinline__ int clamp_index(double f, int hi) {
   if (!(f > 0.0)) return 0;  // also maps NaN to 0
   if (f > (double)hi) return hi;
@@ -76,12 +94,12 @@ __device__ __forceinline__ double sample_field(const T* __restrict__ f, int fh,
   const int y1 = clamp_index(ceil(y), fh - 1);
   const double tx = x - (double)x0;
   const double ty = y - (double)y0;
-  const double c00 = (double)f[(size_t)y0 * fw + x0];
-  const double c10 = (double)f[(size_t)y0 * fw + x1];
-  const double c01 = (double)f[(size_t)y1 * fw + x0];
-  const double c11 = (double)f[(size_t)y1 * fw + x1];
-  const double c0 = c00 + (c10 - c00) * tx;
-  const double c1 = c01 + (c11 - c01) * tx;
+  const T c00 = f[(size_t)y0 * fw + x0];
+  const T c10 = f[(size_t)y0 * fw + x1];
+  const T c01 = f[(size_t)y1 * fw + x0];
+  const T c11 = f[(size_t)y1 * fw + x1];
+  const double c0 = (double)c00 + (double)(T)(c10 - c00) * tx;
+  const double c1 = (double)c01 + (double)(T)(c11 - c01) * tx;
   return c0 + (c1 - c0) * ty;
 }
 

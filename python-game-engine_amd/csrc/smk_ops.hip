@@ -68,21 +68,43 @@ __global__ void k_is_fluid(const uint8_t* __restrict__ fmu, const uint8_t* __res
                             fmv[(size_t)(j + 1) * W + i]) ? 1 : 0;
 }
 
+// The flags of cell (i, j) from the face masks, both fluid bits set alike.
+__device__ __forceinline__ uint8_t cell_flags_at(const uint8_t* __restrict__ fmu,
+                                                 const uint8_t* __restrict__ fmv, int H, int W,
+                                                 int i, int j) {
+  const size_t u0 = (size_t)j * (W + 1) + i;
+  const uint8_t ml = fmu[u0], mr = fmu[u0 + 1];
+  const uint8_t md = fmv[(size_t)j * W + i], mu = fmv[(size_t)(j + 1) * W + i];
+  uint8_t f = (ml || mr || md || mu) ? (uint8_t)(F_FLUID | F_LFLUID) : (uint8_t)0;
+  if (ml && i > 0) f |= F_LEFT;
+  if (mr && i < W - 1) f |= F_RIGHT;
+  if (md && j > 0) f |= F_DOWN;
+  if (mu && j < H - 1) f |= F_UP;
+  return f;
+}
+
 __global__ void k_cell_flags(const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv,
                              int H, int W, uint8_t* __restrict__ flags) {
   int bx_, by_;
   xcd_tile(bx_, by_);
   const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
-  const size_t u0 = (size_t)j * (W + 1) + i;
-  const uint8_t ml = fmu[u0], mr = fmu[u0 + 1];
-  const uint8_t md = fmv[(size_t)j * W + i], mu = fmv[(size_t)(j + 1) * W + i];
-  uint8_t f = (ml || mr || md || mu) ? F_FLUID : 0;
-  if (ml && i > 0) f |= F_LEFT;
-  if (mr && i < W - 1) f |= F_RIGHT;
-  if (md && j > 0) f |= F_DOWN;
-  if (mu && j < H - 1) f |= F_UP;
-  flags[(size_t)j * W + i] = f;
+  flags[(size_t)j * W + i] = cell_flags_at(fmu, fmv, H, W, i, j);
+}
+
+// Refresh the smoother's bits (F_FLUID and the neighbour bits) from the
+// current face masks and keep the level's cached F_LFLUID: the state of a
+// cached multigrid level 0 whose face-mask arrays changed in place
+// (solver.py:42-47 keeps the arrays by reference and is_fluid by value).
+__global__ void k_refresh_flags(const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv,
+                                int H, int W, uint8_t* __restrict__ flags) {
+  int bx_, by_;
+  xcd_tile(bx_, by_);
+  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
+  if (j >= H || i >= W) return;
+  const size_t o = (size_t)j * W + i;
+  const uint8_t fresh = cell_flags_at(fmu, fmv, H, W, i, j);
+  flags[o] = (uint8_t)((fresh & (uint8_t)~F_LFLUID) | (flags[o] & F_LFLUID));
 }
 
 __global__ void k_coarsen_solid(const uint8_t* __restrict__ sf, int H, int W,
@@ -551,7 +573,7 @@ __global__ __launch_bounds__(BT_NT) void k_b_divergence_norm(const BatchScene* _
       const double dvg = du + dv;
       d.div[b] = dvg;
       d.p[b] = 0.0;
-      if (d.flags0[b] & F_FLUID) {
+      if (d.flags0[b] & F_LFLUID) {
         const double bb = d.alpha0 * dvg;
         acc = bb * bb;
       }
@@ -644,6 +666,12 @@ int batch_gradient(const BatchScene* sc, const int4* jobs, int njobs, int* parit
   return check_launch("k_b_gradient");
 }
 
+int refresh_cell_flags(const uint8_t* fmu, const uint8_t* fmv, int H, int W, uint8_t* flags,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_refresh_flags, grid2d(H, W), dim3(BX, BY), 0, s, fmu, fmv, H, W, flags);
+  return check_launch("k_refresh_flags");
+}
+
 }  // namespace smk
 
 using namespace smk;
@@ -662,7 +690,7 @@ using namespace smk;
 extern "C" {
 
 const char* smk_last_error(void) { return g_last_error.c_str(); }
-int smk_abi_version(void) { return 4; }
+int smk_abi_version(void) { return 5; }
 
 int smk_face_masks_from_solid(const uint8_t* solid, int H, int W, uint8_t* fmu,
                               uint8_t* fmv, void* stream) {

@@ -174,7 +174,7 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
         corr[k] = (T)sample_field(a.coarse_e, a.Hc, a.Wc, xc - 0.5, yc - 0.5);
       }
 #pragma unroll
-      for (int k = 0; k < KH; ++k) pv[k] = pv[k] + ((fv[k] & F_FLUID) ? corr[k] : (T)0);
+      for (int k = 0; k < KH; ++k) pv[k] = pv[k] + ((fv[k] & F_LFLUID) ? corr[k] : (T)0);
     }
 #pragma unroll
     for (int k = 0; k < KH; ++k) {
@@ -276,7 +276,7 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
           T ce, ns, b;
           stencil_load(cur, sr, o, RW, f, ce, ns, b);
           const T rv = (T)residual_value(ce, ns, b, a.alpha);
-          const bool fl = f & F_FLUID;
+          const bool fl = f & F_LFLUID;
           total += fl ? (double)rv : 0.0;
           count += fl ? 1 : 0;
         }
@@ -296,7 +296,7 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
       T ce, ns, b;
       stencil_load(cur, sr, o, RW, f, ce, ns, b);
       const double rv = residual_value(ce, ns, b, a.alpha);
-      acc += (xout && gy < H && (f & F_FLUID)) ? rv * rv : 0.0;
+      acc += (xout && gy < H && (f & F_LFLUID)) ? rv * rv : 0.0;
     }
     const double tot = block_sum<NT>(acc, red);
     if (threadIdx.x == 0) a.norm_partial[part] = tot;
@@ -384,7 +384,7 @@ __device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int
                                              const uint32_t (&fv)[KH][2], T* sp, uint8_t* sf,
                                              RTile<T, RH, NT>& t, const T* spatch) {
   using RT = RTile<T, RH, NT>;
-  constexpr uint32_t F_ALL = F_FLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
+  constexpr uint32_t F_ALL = F_FLUID | F_LFLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
   const int R = a.R;
   const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -398,7 +398,7 @@ __device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int
       for (int q = 0; q < 2; ++q) {
         const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
         const T corr = prolong_at<T, RH, NT>(a, spatch, bx, by, gx, gy);
-        pv[k][q] = (fv[k][q] & F_FLUID) ? pv[k][q] + corr : pv[k][q];
+        pv[k][q] = (fv[k][q] & F_LFLUID) ? pv[k][q] + corr : pv[k][q];
       }
   }
 #pragma unroll
@@ -579,7 +579,7 @@ __device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTi
         const T d = fbit(fwc, k, F_DOWN) ? dv : ce;
         const T u = fbit(fwc, k, F_UP) ? uv : ce;
         ns = l + r + d + u;
-        fl = fbit(fwc, k, F_FLUID);
+        fl = fbit(fwc, k, F_LFLUID);
       }
       const double rr = (sizeof(T) == 4 && a.fast_math)
                             ? (double)residual_f((float)ce, (float)ns, (float)t.rb[c][k],
@@ -619,7 +619,7 @@ __device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTi
 #pragma unroll
         for (int di = 0; di < 2; ++di) {
           const int o = (R + 2 * cy + dj) * RW + R + 2 * cx + di;
-          const bool fl = sf[o] & F_FLUID;
+          const bool fl = sf[o] & F_LFLUID;
           const T rv = sp[o];
           total += fl ? (double)rv : 0.0;
           count += fl ? 1 : 0;
@@ -678,8 +678,8 @@ __device__ __forceinline__ void cpatch_store(const T (&v)[CPatch<T, RH, NT>::PER
 // The bilinear prolongation of the staged coarse patch at fine cell (gx, gy).
 // The sample position of fine index g is g / 2 - 0.25, never an integer, so
 // sample_field's floor and ceil are (g - 1) >> 1 and one more; with the same
-// clamping, weights (x minus the clamped floor, exact) and lerps the value is
-// sample_field's.
+// clamping, weights (x minus the clamped floor, exact) and lerps (the first
+// two differences in T) the value is sample_field's.
 template <typename T, int RH, int NT>
 __device__ __forceinline__ T prolong_at(const SmoothArgs<T>& a, const T* spatch, int bx, int by,
                                         int gx, int gy) {
@@ -692,10 +692,10 @@ __device__ __forceinline__ T prolong_at(const SmoothArgs<T>& a, const T* spatch,
   const double tx = ((gx + 0.5) * 0.5 - 0.5) - (double)x0;
   const double ty = ((gy + 0.5) * 0.5 - 0.5) - (double)y0;
   const int r0 = (y0 - cy0) * PW, r1 = (y1 - cy0) * PW;
-  const double c00 = (double)spatch[r0 + x0 - cx0], c10 = (double)spatch[r0 + x1 - cx0];
-  const double c01 = (double)spatch[r1 + x0 - cx0], c11 = (double)spatch[r1 + x1 - cx0];
-  const double c0 = c00 + (c10 - c00) * tx;
-  const double c1 = c01 + (c11 - c01) * tx;
+  const T c00 = spatch[r0 + x0 - cx0], c10 = spatch[r0 + x1 - cx0];
+  const T c01 = spatch[r1 + x0 - cx0], c11 = spatch[r1 + x1 - cx0];
+  const double c0 = (double)c00 + (double)(T)(c10 - c00) * tx;
+  const double c1 = (double)c01 + (double)(T)(c11 - c01) * tx;
   return (T)(c0 + (c1 - c0) * ty);
 }
 
@@ -719,7 +719,7 @@ __device__ __forceinline__ void rtile_prolong(const SmoothArgs<T>& a, const RTil
       const T corr = prolong_at<T, RH, NT>(a, spatch, bx, by, gx, gy0 + ly);
       // colour of row 2m + q is (q + x) & 1; select the word, not the array
       const uint32_t fwd = ((q + px) & 1) ? t.fw[1][k >> 2] : t.fw[0][k >> 2];
-      const bool fl = ((fwd >> (8 * (k & 3))) & F_FLUID) != 0u;
+      const bool fl = ((fwd >> (8 * (k & 3))) & F_LFLUID) != 0u;
       const int o = ly * RW + lane;
       sp[o] = fl ? sp[o] + corr : sp[o];
     }
@@ -802,7 +802,7 @@ __device__ __forceinline__ void rtile_stage_rf(const SmoothArgs<T>& a, int bx, i
                                                uint8_t* sf, RTile<T, RH, NT>& t) {
   using RT = RTile<T, RH, NT>;
   constexpr int K = RT::K;
-  constexpr uint32_t F_ALL = F_FLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
+  constexpr uint32_t F_ALL = F_FLUID | F_LFLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
   const int R = a.R;
   const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1067,7 +1067,7 @@ __global__ __launch_bounds__(NW * 64) void k_rb_reg(const SmoothArgs<T> a) {
                                   (gy + 0.5) * 0.5 - 0.5);
       }
 #pragma unroll
-      for (int k = 0; k < HR; ++k) pv[k] = pv[k] + ((fv[k] & F_FLUID) ? corr[k] : (T)0);
+      for (int k = 0; k < HR; ++k) pv[k] = pv[k] + ((fv[k] & F_LFLUID) ? corr[k] : (T)0);
     }
 #pragma unroll
     for (int k = 0; k < HR; ++k) {
@@ -1189,14 +1189,14 @@ __global__ __launch_bounds__(NW * 64) void k_rb_reg(const SmoothArgs<T> a) {
       const uint32_t f01 = (uint32_t)from_right((int)flo), f11 = (uint32_t)from_right((int)fhi);
       double total = 0.0;
       int count = 0;
-      total += (flo & F_FLUID) ? (double)rlo : 0.0;
-      count += (flo & F_FLUID) ? 1 : 0;
-      total += (f01 & F_FLUID) ? (double)r01 : 0.0;
-      count += (f01 & F_FLUID) ? 1 : 0;
-      total += (fhi & F_FLUID) ? (double)rhi : 0.0;
-      count += (fhi & F_FLUID) ? 1 : 0;
-      total += (f11 & F_FLUID) ? (double)r11 : 0.0;
-      count += (f11 & F_FLUID) ? 1 : 0;
+      total += (flo & F_LFLUID) ? (double)rlo : 0.0;
+      count += (flo & F_LFLUID) ? 1 : 0;
+      total += (f01 & F_LFLUID) ? (double)r01 : 0.0;
+      count += (f01 & F_LFLUID) ? 1 : 0;
+      total += (fhi & F_LFLUID) ? (double)rhi : 0.0;
+      count += (fhi & F_LFLUID) ? 1 : 0;
+      total += (f11 & F_LFLUID) ? (double)r11 : 0.0;
+      count += (f11 & F_LFLUID) ? 1 : 0;
       const int Jc = gy >> 1, Ic = gx >> 1;
       if (px == 0 && xout && yin && Jc < Hc && Ic < Wc) {
         const T mean = count > 0 ? (T)(total / (double)count) : (T)0;
@@ -1207,8 +1207,8 @@ __global__ __launch_bounds__(NW * 64) void k_rb_reg(const SmoothArgs<T> a) {
       // each row of the pair against the output window on its own: with an
       // odd halo a pair straddles the window's edge
       const bool yin1 = ly + 1 >= R && ly + 1 < R + TY;
-      const bool in0 = xout && yin && gy < H && (flo & F_FLUID);
-      const bool in1 = xout && yin1 && gy + 1 < H && (fhi & F_FLUID);
+      const bool in0 = xout && yin && gy < H && (flo & F_LFLUID);
+      const bool in1 = xout && yin1 && gy + 1 < H && (fhi & F_LFLUID);
       acc += in0 ? dlo * dlo : 0.0;
       acc += in1 ? dhi * dhi : 0.0;
     }
@@ -1798,7 +1798,7 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
           const T dn = (f & F_DOWN) ? p[o >= W ? o - W : 0] : ce;
           const T up = (f & F_UP) ? p[o + W < N ? o + W : N - 1] : ce;
           const T rv = (T)residual_value(ce, (T)(lf + rt + dn + up), SB(l)[o], a.alpha[l]);
-          const bool fl = f & F_FLUID;
+          const bool fl = f & F_LFLUID;
           total += fl ? (double)rv : 0.0;  // +0.0 for a non-fluid child is exact
           count += fl ? 1 : 0;
         }
@@ -1833,7 +1833,7 @@ __device__ __forceinline__ void vcycle_small_body(const SmallArgs<T>& a) {
     for (int o = threadIdx.x; o < H * W; o += NT) {
       const int j = o / W, i = o - j * W;
       T corr = (T)0;
-      if (SF(l)[o] & F_FLUID)
+      if (SF(l)[o] & F_LFLUID)
         corr = (T)sample_field(SP(l + 1), Hc, Wc, (i + 0.5) * 0.5 - 0.5, (j + 0.5) * 0.5 - 0.5);
       SP(l)[o] = SP(l)[o] + corr;
     }
@@ -1897,7 +1897,7 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
     const size_t o = (size_t)j * W + i;
     const uint8_t f = flags[o];
     double rv = 0.0;
-    if (f & F_FLUID) {
+    if (f & F_LFLUID) {
       const T ce = p[o];
       const T l = (f & F_LEFT) ? p[o - 1] : ce;
       const T rr = (f & F_RIGHT) ? p[o + 1] : ce;
@@ -1906,7 +1906,7 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
       rv = poisson_residual(l, rr, d, u, ce, rhs[o], alpha);
       acc = rv * rv;
     }
-    if (defect != nullptr) defect[o] = (f & F_FLUID) ? (float)(rv / alpha) : 0.0f;
+    if (defect != nullptr) defect[o] = (f & F_LFLUID) ? (float)(rv / alpha) : 0.0f;
   }
   const double tot = block_sum<NB_T>(acc, red);
   if (threadIdx.x == 0) partial[by * gridDim.x + bx] = tot;
@@ -1941,7 +1941,7 @@ __global__ __launch_bounds__(NB_T) void k_correct_defect(const T* __restrict__ p
     const T ce = q(o);
     p_out[o] = ce;
     double rv = 0.0;
-    if (f & F_FLUID) {
+    if (f & F_LFLUID) {
       const T l = (f & F_LEFT) ? q(o - 1) : ce;
       const T rr = (f & F_RIGHT) ? q(o + 1) : ce;
       const T d = (f & F_DOWN) ? q(o - W) : ce;
@@ -1949,7 +1949,7 @@ __global__ __launch_bounds__(NB_T) void k_correct_defect(const T* __restrict__ p
       rv = poisson_residual(l, rr, d, u, ce, rhs[o], alpha);
       acc = rv * rv;
     }
-    defect[o] = (f & F_FLUID) ? (float)(rv / alpha) : 0.0f;
+    defect[o] = (f & F_LFLUID) ? (float)(rv / alpha) : 0.0f;
   }
   const double tot = block_sum<NB_T>(acc, red);
   if (threadIdx.x == 0) partial[by * gridDim.x + bx] = tot;
@@ -1971,7 +1971,7 @@ __global__ __launch_bounds__(NB_T) void k_rhs_norm(const T* __restrict__ rhs,
   double acc = 0.0;
   if (i < W && j < H) {
     const size_t o = (size_t)j * W + i;
-    const bool fl = (flags[o] & F_FLUID) != 0;
+    const bool fl = (flags[o] & F_LFLUID) != 0;
     if (fl) {
       const double b = alpha * (double)rhs[o];
       acc = b * b;
@@ -2036,7 +2036,10 @@ __global__ void k_state_init(SolveState* __restrict__ st) {
 //
 // The resulting state is also written to *host_out (a slot of a pinned host
 // ring, may be null) with sequence number seq, so the host learns the
-// outcome without a copy or an event of its own on the stream.
+// outcome without a copy or an event of its own on the stream.  A check
+// queued after the solve has ended (state inactive) publishes nothing: the
+// host never waits for it, and it may run after the ring has been handed to
+// a later solve.
 __device__ __forceinline__ void publish(RingSlot* slot, const SolveState& v, unsigned seq) {
   slot->active = v.active;
   slot->cycles = v.cycles;
@@ -2051,10 +2054,7 @@ __global__ __launch_bounds__(1024) void k_check(const double* __restrict__ parti
                                                 int cap, int inner,
                                                 RingSlot* __restrict__ host_out, unsigned seq) {
   __shared__ double red[1024 / 64];
-  if (st->active == 0) {
-    if (threadIdx.x == 0 && host_out != nullptr) publish(host_out, *st, seq);
-    return;
-  }
+  if (st->active == 0) return;
   double acc = 0.0;
   for (int k = threadIdx.x; k < n; k += 1024) acc += partial[k];
   const double tot = block_sum<1024>(acc, red);
@@ -2072,7 +2072,7 @@ __global__ __launch_bounds__(1024) void k_check(const double* __restrict__ parti
 __global__ void k_solid_from_flags(const uint8_t* __restrict__ flags, long n,
                                    uint8_t* __restrict__ solid) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) solid[k] = (flags[k] & F_FLUID) ? 0 : 1;
+  if (k < n) solid[k] = (flags[k] & F_LFLUID) ? 0 : 1;
 }
 
 // ------------------------------------------------------------------ host side
@@ -3694,6 +3694,37 @@ int smk_residual_norm2(const void* p, const void* div, const uint8_t* fmu, const
   return SMK_OK;
 }
 
+// The masks of every level from the level-0 face masks (solver.py:34-73):
+// level-0 flags, solid = ~is_fluid (solver.py:38-40), then per coarse level
+// the coarsened solid, its face masks and flags.  In place, stream-ordered.
+static int mg_build_masks(smk_mg* mg, const uint8_t* fmu, const uint8_t* fmv, hipStream_t s) {
+  int rc = smk_cell_flags(fmu, fmv, mg->H, mg->W, mg->lv[0].flags, s);
+  if (!rc) {
+    const long n = (long)mg->H * mg->W;
+    hipLaunchKernelGGL(k_solid_from_flags, dim3(cdiv(n, 256)), dim3(256), 0, s,
+                       mg->lv[0].flags, n, mg->lv[0].solid);
+    rc = check_launch("k_solid_from_flags");
+  }
+  for (int l = 1; l < mg->nl && !rc; ++l) {
+    Level& F = mg->lv[l - 1];
+    Level& C = mg->lv[l];
+    rc = smk_coarsen_solid(F.solid, F.H, F.W, C.solid, s);
+    if (!rc) rc = smk_face_masks_from_solid(C.solid, C.H, C.W, C.fmu, C.fmv, s);
+    if (!rc) rc = smk_cell_flags(C.fmu, C.fmv, C.H, C.W, C.flags, s);
+  }
+  mg->cop_npass = -1;  // the coarsest operator follows the coarsest flags
+  return rc;
+}
+
+int smk_mg_refresh_masks(smk_mg* mg, const uint8_t* fmu, const uint8_t* fmv, int all_levels,
+                         void* stream) {
+  SMK_REQUIRE(mg && fmu && fmv, "smk_mg_refresh_masks: null pointer");
+  hipStream_t s = as_stream(stream);
+  if (all_levels) return mg_build_masks(mg, fmu, fmv, s);
+  if (mg->nl == 1) mg->cop_npass = -1;  // the coarsest level is level 0
+  return refresh_cell_flags(fmu, fmv, mg->H, mg->W, mg->lv[0].flags, s);
+}
+
 int smk_mg_destroy(smk_mg* mg) {
   if (mg == nullptr) return SMK_OK;
   for (int l = 0; l < mg->nl; ++l) {
@@ -3773,21 +3804,7 @@ int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W, int num_
     }
     if (e != hipSuccess) rc = fail((int)e, "smk_mg_create: %s", hipGetErrorString(e));
   }
-  // level 0 flags from the caller's masks; solid = ~is_fluid (solver.py:38-40)
-  if (!rc) rc = smk_cell_flags(fmu, fmv, H, W, mg->lv[0].flags, stream);
-  if (!rc) {
-    const long n = (long)H * W;
-    hipLaunchKernelGGL(k_solid_from_flags, dim3(cdiv(n, 256)), dim3(256), 0, s,
-                       mg->lv[0].flags, n, mg->lv[0].solid);
-    rc = check_launch("k_solid_from_flags");
-  }
-  for (int l = 1; l < mg->nl && !rc; ++l) {
-    Level& F = mg->lv[l - 1];
-    Level& C = mg->lv[l];
-    rc = smk_coarsen_solid(F.solid, F.H, F.W, C.solid, stream);
-    if (!rc) rc = smk_face_masks_from_solid(C.solid, C.H, C.W, C.fmu, C.fmv, stream);
-    if (!rc) rc = smk_cell_flags(C.fmu, C.fmv, C.H, C.W, C.flags, stream);
-  }
+  if (!rc) rc = mg_build_masks(mg, fmu, fmv, s);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = fail((int)e, "smk_mg_create: %s", hipGetErrorString(e));

@@ -315,6 +315,50 @@ def test_two_level_vcycle_is_dense_composition(kind, scale, smoother):
     close(out.ravel(), p, 1e-11)
 
 
+@pytest.mark.parametrize("smoother", ["rb_sor_gauss_seidel", "jacobi"])
+def test_two_level_vcycle_with_cached_level0(smoother):
+    """The reference's cache after the face-mask arrays changed in place
+    without a rebuild (solver.py:42-47, :87-92): the level-0 smoother derives
+    is_fluid from the current masks, while the level-0 residual, restriction
+    and prolongation read the is_fluid of the build and the coarse level is
+    built from it.  Obstacles both appear and disappear between the two mask
+    sets, so using either is_fluid everywhere fails."""
+    H, W = 8, 10
+    _, bfmu, bfmv, _ = masks(H, W, "obstacles", seed=3)
+    _, fmu, fmv, _ = masks(H, W, "obstacles", seed=11)
+    stale = _dense_level(bfmu, bfmv)["fl"]
+    cur = _dense_level(fmu, fmv)
+    assert (stale & ~cur["fl"]).any() and (cur["fl"] & ~stale).any()
+    lvl = dict(cur, fl=stale)  # level 0 as the residual and transfers see it
+    sc = O.coarsen_solid_mask(~stale.reshape(H, W))
+    L1 = _dense_level(*O.build_face_masks(sc))
+    rng = np.random.default_rng(21)
+    div = rng.standard_normal((H, W))
+    p0 = rng.standard_normal((H, W))
+    h, rho, dt, om = 0.25, 1.3, 0.02, 1.0
+    a0, a1 = O.calculate_alpha(h, rho, dt), O.calculate_alpha(2 * h, rho, dt)
+    rhs0 = div.ravel()
+    smooth = {"rb_sor_gauss_seidel": lambda L, p, r, a, k: _dense_rb(L, p, r, a, om, k),
+              "jacobi": _dense_jacobi}[smoother]
+    p = smooth(cur, p0.ravel(), rhs0, a0, 2)
+    A0 = 4 * np.eye(H * W) - cur["N"]
+    r = np.where(stale, a0 * rhs0 - A0 @ p, 0.0)
+    rc = _restriction_matrix(lvl) @ r / a1
+    e = smooth(L1, np.zeros(rc.size), rc, a1, 50)
+    p = p + _prolongation_matrix(lvl, H // 2, W // 2) @ e
+    p = smooth(cur, p, rhs0, a0, 2)
+    out = p0.copy()
+    O.multigrid_solver(out, div, fmu, fmv, rho, h, dt, v_cycles=1, smoother_type=smoother,
+                       num_levels=2, smoother_iterations=2, coarsest_iterations=50,
+                       smoother_omega=om, level_is_fluid=stale.reshape(H, W))
+    close(out.ravel(), p, 1e-11)
+    fresh = p0.copy()
+    O.multigrid_solver(fresh, div, fmu, fmv, rho, h, dt, v_cycles=1, smoother_type=smoother,
+                       num_levels=2, smoother_iterations=2, coarsest_iterations=50,
+                       smoother_omega=om)
+    assert np.abs(fresh - out).max() > 1e-6  # the cache state matters
+
+
 def test_reference_coarse_correction_is_quarter_scaled():
     """The claim behind coarse_scale: with an exact coarse solve the reference
     scaling gives e_c = A_c^+ R r, a quarter of the rediscretised
