@@ -58,7 +58,7 @@ _SIGS = {
     "orc_prolong": (None, [_dp, _i, _i, _u8, _i, _i, _dp]),
     "orc_mg_solve": (_i, [_dp, _dp, _u8, _u8, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i, _d, _i, _d, _dp]),
     "orc_mg_solve_cached": (_i, [_dp, _dp, _u8, _u8, _u8, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i,
-                                 _d, _i, _d, _dp]),
+                                 _d, _d, _i, _d, _dp]),
     "orc_falloff": (_d, [_d, _d, _i]),
     "orc_emit_scalar": (None, [_dp, _i, _i, _d, _d, _d, _d, _d, _i]),
     "orc_num_threads": (_i, []),
@@ -311,7 +311,7 @@ def multigrid_solver(p, div, face_mask_u, face_mask_v, fluid_density, cell_size,
                      dt, v_cycles=2, smoother_type="jacobi", num_levels=4,
                      smoother_iterations=2, coarsest_iterations=50,
                      smoother_omega=0.0, coarse_scale="reference", tolerance=0.0,
-                     level_is_fluid=None, **_ignored):
+                     level_is_fluid=None, coarsest_omega=0.0, **_ignored):
     """multigrid/solver.py:141-176 (fresh cache per call).
 
     ``coarse_scale="reference"`` keeps solver.py:124-125 (restrict(r)/alpha(2h));
@@ -322,6 +322,10 @@ def multigrid_solver(p, div, face_mask_u, face_mask_v, fluid_density, cell_size,
     are the arrays that cache holds, changed in place since: the smoothers
     see the masks, the level-0 residual, restriction and prolongation and
     the coarse levels the cached is_fluid.  None: a fresh cache.
+
+    ``coarsest_omega`` (new): the red-black omega of the coarsest level only,
+    > 0 as given, < 0 the optimal omega of that level's size, 0 (the
+    reference) the other levels' omega.
 
     Returns (v-cycles run, relative residual or -1 when no tolerance)."""
     H, W = p.shape
@@ -334,6 +338,7 @@ def multigrid_solver(p, div, face_mask_u, face_mask_v, fluid_density, cell_size,
                                   cell_size, dt, int(v_cycles), SMOOTHERS[smoother_type],
                                   int(num_levels), int(smoother_iterations),
                                   int(coarsest_iterations), float(smoother_omega),
+                                  float(coarsest_omega),
                                   1 if coarse_scale == "rediscretized" else 0, float(tolerance),
                                   ctypes.byref(rel))
     return n, rel.value

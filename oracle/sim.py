@@ -142,7 +142,8 @@ class OracleSmokeSim:
                 coarsest_iterations=sp.get("coarsest_iterations", 30),
                 smoother_omega=sp.get("smoother_omega", 0.0),
                 coarse_scale=sp.get("coarse_scale", "reference"), tolerance=sp.get("tolerance", 0.0),
-                level_is_fluid=self.mg_level_fluid)
+                level_is_fluid=self.mg_level_fluid,
+                coarsest_omega=sp.get("coarsest_omega", 0.0))
             self.last_solve = {"iterations": n, "rel_residual": rel}
         else:
             raise ValueError(self.solver)

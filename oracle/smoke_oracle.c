@@ -543,13 +543,22 @@ static void mg_smooth(orc_level* L, double rho, double h, double dt, int smoothe
   }
 }
 
+/* The omega of the coarsest level's red-black smoothing: coarsest_omega > 0
+ * as given, < 0 the optimal omega of that level's size (common.py:30-39),
+ * 0 the other levels' (a new option; 0 is the reference's behaviour). */
+static double coarsest_om(const orc_level* L, double om, double coarsest_omega) {
+  if (coarsest_omega > 0.0) return coarsest_omega;
+  if (coarsest_omega < 0.0) return orc_optimal_omega(L->H > L->W ? L->H : L->W);
+  return om;
+}
+
 static void mg_vcycle(orc_level* lv, int nl, int l, double rho, double h,
                       double dt, int smoother, int si, int ci, double om,
-                      int rediscretize) {
+                      int rediscretize, double com) {
   orc_level* L = &lv[l];
   const int coarsest = (l == nl - 1);
   const int n_iter = coarsest ? ci : si;
-  mg_smooth(L, rho, h, dt, smoother, n_iter, om);
+  mg_smooth(L, rho, h, dt, smoother, n_iter, coarsest ? coarsest_om(L, om, com) : om);
   if (coarsest) return;
   const double alpha = orc_alpha(h, rho, dt);
   orc_residual(L->p, L->rhs, L->fl, L->fmu, L->fmv, L->H, L->W, alpha, 0.25, L->r);
@@ -563,7 +572,7 @@ static void mg_vcycle(orc_level* lv, int nl, int l, double rho, double h,
   const size_t nc = (size_t)C->H * C->W;
   for (size_t k = 0; k < nc; ++k) C->rhs[k] /= alpha_c; /* solver.py:125 */
   memset(C->p, 0, nc * sizeof(double));
-  mg_vcycle(lv, nl, l + 1, rho, h * 2.0, dt, smoother, si, ci, om, rediscretize);
+  mg_vcycle(lv, nl, l + 1, rho, h * 2.0, dt, smoother, si, ci, om, rediscretize, com);
   orc_prolong(C->p, C->H, C->W, L->fl, L->H, L->W, L->corr);
   const size_t n = (size_t)L->H * L->W;
   for (size_t k = 0; k < n; ++k) L->p[k] += L->corr[k]; /* solver.py:134 */
@@ -578,8 +587,8 @@ int orc_mg_solve(double* p, const double* div, const uint8_t* fmu,
                  int rediscretize, double tol, double* rel_out) {
   return orc_mg_solve_cached(p, div, fmu, fmv, NULL, H, W, rho, cell_size, dt,
                              v_cycles, smoother, num_levels, smoother_iterations,
-                             coarsest_iterations, smoother_omega, rediscretize,
-                             tol, rel_out);
+                             coarsest_iterations, smoother_omega, 0.0,
+                             rediscretize, tol, rel_out);
 }
 
 /* As orc_mg_solve, with the level cache of solver.py:87-92 left in place
@@ -596,8 +605,8 @@ int orc_mg_solve_cached(double* p, const double* div, const uint8_t* fmu,
                         int W, double rho, double cell_size, double dt,
                         int v_cycles, int smoother, int num_levels,
                         int smoother_iterations, int coarsest_iterations,
-                        double smoother_omega, int rediscretize, double tol,
-                        double* rel_out) {
+                        double smoother_omega, double coarsest_omega,
+                        int rediscretize, double tol, double* rel_out) {
   if (num_levels < 1) num_levels = 1;
   orc_level* lv = (orc_level*)calloc((size_t)num_levels, sizeof(orc_level));
   /* solver.py:34-73 _build_levels */
@@ -650,7 +659,7 @@ int orc_mg_solve_cached(double* p, const double* div, const uint8_t* fmu,
     rel = b2 > 0.0 ? sqrt(orc_norm2_fluid(lv[0].r, lv[0].fl, H, W) / b2) : 0.0;
     while (rel > tol && cycles < v_cycles) {
       mg_vcycle(lv, nl, 0, rho, cell_size, dt, smoother, smoother_iterations,
-                coarsest_iterations, smoother_omega, rediscretize);
+                coarsest_iterations, smoother_omega, rediscretize, coarsest_omega);
       cycles++;
       orc_residual(lv[0].p, lv[0].rhs, lv[0].fl, fmu, fmv, H, W, alpha, 0.25, lv[0].r);
       rel = b2 > 0.0 ? sqrt(orc_norm2_fluid(lv[0].r, lv[0].fl, H, W) / b2) : 0.0;
@@ -658,7 +667,7 @@ int orc_mg_solve_cached(double* p, const double* div, const uint8_t* fmu,
   } else {
     for (int c = 0; c < v_cycles; ++c) {
       mg_vcycle(lv, nl, 0, rho, cell_size, dt, smoother, smoother_iterations,
-                coarsest_iterations, smoother_omega, rediscretize);
+                coarsest_iterations, smoother_omega, rediscretize, coarsest_omega);
       cycles++;
     }
   }

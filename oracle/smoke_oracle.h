@@ -149,8 +149,8 @@ int orc_mg_solve_cached(double* p, const double* div, const uint8_t* fmu,
                         int W, double rho, double cell_size, double dt,
                         int v_cycles, int smoother, int num_levels,
                         int smoother_iterations, int coarsest_iterations,
-                        double smoother_omega, int rediscretize, double tol,
-                        double* rel_out);
+                        double smoother_omega, double coarsest_omega,
+                        int rediscretize, double tol, double* rel_out);
 
 /* emitters.py:6-35 falloff, approach 0 hard_cutoff, 1 linear, 2 quadratic,
  * 3 gaussian */

@@ -135,3 +135,87 @@ def test_mixed_rbsor_matches_fp64_to_tolerance(N):
     assert abs(i32["iterations"] - i64["iterations"]) <= 0.1 * i64["iterations"] + 40
     a, b = p64 - p64.mean(), p32 - p32.mean()
     assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()
+
+
+# the bench line's headline variant: two FP32 V-cycles per FP64 refinement,
+# the coarsest level applied as its operator, float32 pass arithmetic
+HEADLINE_MIXED = {"precision": "mixed", "inner_cycles": 2, "coarsest_solve": "operator",
+                  "inner_arithmetic": "float32"}
+
+
+@pytest.mark.parametrize("tol,bound", [(1e-8, 1e-6), (1e-11, 1e-9)])
+def test_headline_mixed_variant_against_fp64_oracle(tol, bound):
+    """The headline configuration of the 2048x2048 workload at 128x128, 5
+    steps, against the all-FP64 oracle rollout solved to the same tolerance.
+    Every solve reaches the tolerance in the FP64 residual.  At the
+    workload's 1e-8 the fields differ from the oracle's by ~1e-7 relative L2
+    (measured 1.6e-7 density, 8e-8 velocity): two solutions each meeting
+    1e-8, not FP32 rounding, since at 1e-11 the difference falls with the
+    tolerance (bound 1e-9)."""
+    sc = scenes.plume_2048_mg(N=128, steps=5)
+    sim = scenes.make_simulation(sc, dtype="float64",
+                                 solver_overrides=dict(HEADLINE_MIXED, tolerance=tol))
+    ref = oracle.sim.from_scene(sc, solver_overrides={"tolerance": tol})
+    for _ in range(sc.steps):
+        sim.substep(sc.dt)
+        ref.substep(sc.dt)
+        info = sim.last_solve_info
+        assert info["converged"] == 1 and info["rel_residual"] <= tol, info
+        assert info["refinements"] >= 1 and info["inner_cycles"] == 2 * info["refinements"]
+    torch.cuda.synchronize()
+    errs = {f: rel(getattr(sim, f).cpu().numpy(), getattr(ref, f))
+            for f in ("density", "velocity_u", "velocity_v")}
+    print(f"headline mixed vs FP64 oracle, 128^2 x 5 steps, tol {tol}:", errs)
+    for f, e in errs.items():
+        assert e <= bound, (f, e)
+
+
+def test_headline_mixed_full_size_step_vs_fp64():
+    """One 2048x2048 step of the headline variant against the FP64 V-cycle
+    step on the GPU: both reach 1e-8, and density and velocity agree to
+    1e-6 relative L2, pressure modulo its constant to 1e-5."""
+    sc = scenes.plume_2048_mg(N=2048, steps=1)
+    out = {}
+    for name, ov in (("fp64", None), ("mixed", HEADLINE_MIXED)):
+        sim = scenes.make_simulation(sc, dtype="float64", solver_overrides=ov)
+        sim.substep(sc.dt)
+        torch.cuda.synchronize()
+        info = dict(sim.last_solve_info)
+        assert info["converged"] == 1 and info["rel_residual"] <= 1e-8, (name, info)
+        out[name] = {f: getattr(sim, f).cpu().numpy()
+                     for f in ("density", "velocity_u", "velocity_v", "pressure")}
+        del sim
+    a, b = out["mixed"], out["fp64"]
+    errs = {f: rel(a[f], b[f]) for f in ("density", "velocity_u", "velocity_v")}
+    errs["pressure"] = rel(a["pressure"] - a["pressure"].mean(), b["pressure"] - b["pressure"].mean())
+    print("headline mixed vs FP64 step, 2048^2:", errs)
+    for f in ("density", "velocity_u", "velocity_v"):
+        assert errs[f] <= 1e-6, (f, errs[f])
+    assert errs["pressure"] <= 1e-5
+
+
+def test_plume_512_rbsor_full_size_step_against_oracle():
+    """The 512x512 RB-SOR workload (omega 1.9 to 1e-6) at its full size, one
+    step, FP64 and FP32 sweeps in FP64 defect correction, against the
+    oracle's step.  FP64 runs the same sweeps as the oracle (same count,
+    fields within 1e-12); mixed reaches the tolerance in the same sweeps
+    (one check interval apart at most) with density and velocity within
+    1e-6 relative L2."""
+    sc = scenes.plume_512_rbsor(steps=1)
+    ref = oracle.sim.from_scene(sc)
+    ref.substep(sc.dt)
+    for prec, tol in (("native", 1e-12), ("mixed", 1e-6)):
+        sim = scenes.make_simulation(sc, dtype="float64", solver_overrides={"precision": prec})
+        sim.substep(sc.dt)
+        torch.cuda.synchronize()
+        info = sim.last_solve_info
+        assert info["converged"] == 1 and info["rel_residual"] <= 1e-6, (prec, info)
+        if prec == "native":
+            assert info["iterations"] == ref.last_solve["iterations"]
+        else:
+            assert abs(info["iterations"] - ref.last_solve["iterations"]) <= 20
+        errs = {f: rel(getattr(sim, f).cpu().numpy(), getattr(ref, f))
+                for f in ("density", "velocity_u", "velocity_v")}
+        print(f"512^2 RB-SOR {prec} vs oracle:", info["iterations"], errs)
+        for f, e in errs.items():
+            assert e <= tol, (prec, f, e)

@@ -315,6 +315,34 @@ def test_two_level_vcycle_is_dense_composition(kind, scale, smoother):
     close(out.ravel(), p, 1e-11)
 
 
+@pytest.mark.parametrize("coarsest_omega", [-1.0, 1.6])
+def test_two_level_vcycle_coarsest_omega(coarsest_omega):
+    """coarsest_omega (new option): the coarsest level alone smooths with that
+    omega (< 0: the optimal omega of the coarsest size, common.py:30-39),
+    the fine level with the smoother's."""
+    H, W = 10, 14
+    _, fmu, fmv, _ = masks(H, W, "ring")
+    L0 = _dense_level(fmu, fmv)
+    sc = O.coarsen_solid_mask(~L0["fl"].reshape(H, W))
+    L1 = _dense_level(*O.build_face_masks(sc))
+    rng = np.random.default_rng(5)
+    div = rng.standard_normal((H, W))
+    h, rho, dt, om = 0.5, 1.0, 0.03, 1.0
+    a0 = O.calculate_alpha(h, rho, dt)
+    oc = coarsest_omega if coarsest_omega > 0 else O.calculate_optimal_omega(max(H, W) // 2)
+    p = _dense_rb(L0, np.zeros(H * W), div.ravel(), a0, om, 2)
+    r = np.where(L0["fl"], a0 * div.ravel() - (4 * np.eye(H * W) - L0["N"]) @ p, 0.0)
+    rc = _restriction_matrix(L0) @ r / a0
+    e = _dense_rb(L1, np.zeros(rc.size), rc, O.calculate_alpha(2 * h, rho, dt), oc, 30)
+    p = _dense_rb(L0, p + _prolongation_matrix(L0, H // 2, W // 2) @ e, div.ravel(), a0, om, 2)
+    out = np.zeros((H, W))
+    O.multigrid_solver(out, div, fmu, fmv, rho, h, dt, v_cycles=1,
+                       smoother_type="rb_sor_gauss_seidel", num_levels=2, smoother_iterations=2,
+                       coarsest_iterations=30, smoother_omega=om, coarse_scale="rediscretized",
+                       coarsest_omega=coarsest_omega)
+    close(out.ravel(), p, 1e-11)
+
+
 @pytest.mark.parametrize("smoother", ["rb_sor_gauss_seidel", "jacobi"])
 def test_two_level_vcycle_with_cached_level0(smoother):
     """The reference's cache after the face-mask arrays changed in place
