@@ -170,6 +170,23 @@ int smk_jacobi_solve(void* p, const void* div, const uint8_t* fmu,
                      double cell_size, double dt, int iterations, void* ws,
                      size_t ws_bytes, int dtype, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): smk_jacobi_solve without the workspace argument, same
+ * kernels and results.  Its scratch (cell flags and the p ping-pong buffer)
+ * is a static device array of SMK_NOALLOC_MAX_CELLS cells' worth, 9 MiB,
+ * held on each GPU the library runs on until the process ends; grids of
+ * more cells are rejected with SMK_EINVAL.  The call allocates no device or
+ * pinned memory and only enqueues work on `stream` (no synchronisation),
+ * which must belong to the current device.  Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (on different
+ * streams, or from different threads) race, so order them. */
+#define SMK_NOALLOC_MAX_CELLS (1L << 20)
+int smk_jacobi_solve_noalloc(void* p, const void* div, const uint8_t* fmu,
+                             const uint8_t* fmv, int H, int W, double fluid_density,
+                             double cell_size, double dt, int iterations, int dtype,
+                             void* stream);
+
 /* red_black_sor_gauss_seidel.py:30-42 (solver key "rb_sor_gauss_seidel"):
  * omega <= 0 or > 2 picks calculateOptimalOmega(max(H,W)) (common.py:30-39).
  * tol <= 0: exactly `iterations` sweeps (the reference).  tol > 0: sweeps in

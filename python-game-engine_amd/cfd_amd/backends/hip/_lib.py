@@ -135,6 +135,7 @@ SIGNATURES = {
     "smk_box_source": (_i, [_vp, _i, _i, _i, _i, _i, _i, _d, _i, _i, _vp]),
     "smk_solver_workspace_bytes": (_sz, [_i, _i, _i]),
     "smk_jacobi_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _vp, _sz, _i, _vp]),
+    "smk_jacobi_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _i, _vp]),
     "smk_rbsor_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _d, _d, _i, _vp, _sz,
                              ctypes.POINTER(SolveInfo), _i, _vp]),
     "smk_gs_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _vp, _sz, _i, _vp]),

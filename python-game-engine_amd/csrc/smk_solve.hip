@@ -27,8 +27,10 @@
 // then (sum + alpha*div) * 1/4 and the SOR blend in float64 (Numba types the
 // Python-float coefficients as float64).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "smk_common.hpp"
@@ -2407,7 +2409,7 @@ int launch_small(int mode, const SmallArgs<T>& a, size_t lds, hipStream_t s) {
   return check_launch("k_vcycle_small");
 }
 
-inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 inline int norm_blocks(int H, int W) { return (int)(cdiv(W, NB_X) * cdiv(H, NB_Y)); }
 
@@ -2481,6 +2483,21 @@ int max_passes_per_launch(int mode, int H, int W) {
   return c;
 }
 
+// The pass counts of a run of launches without storing them: nl launches
+// sharing npass passes evenly (the first npass % nl one more), launch
+// `split` (if >= 0) divided into two.
+struct PassChunks {
+  int nl, npass, split;
+  int count() const { return nl + (split >= 0 ? 1 : 0); }
+  int base(int q) const { return npass / nl + (q < npass % nl ? 1 : 0); }
+  int at(int q) const {
+    if (split < 0 || q < split) return base(q);
+    if (q == split) return base(split) / 2;
+    if (q == split + 1) return base(split) - base(split) / 2;
+    return base(q - 1);
+  }
+};
+
 // Runs `npass_total` passes from src into the buffer `dst_final`, using
 // `other` as the ping-pong partner; chunked into launches of at most
 // max_passes_per_launch passes.  The chunking keeps the launch count's
@@ -2498,10 +2515,8 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
     return SMK_OK;
   }
   // as few launches as the cap allows, the passes spread evenly over them
-  std::vector<int> chunks;
   const int cmax = max_passes_per_launch<T>(mode, H, W);
-  const int nl = (npass_total + cmax - 1) / cmax;
-  for (int q = 0; q < nl; ++q) chunks.push_back(npass_total / nl + (q < npass_total % nl ? 1 : 0));
+  PassChunks chunks{(npass_total + cmax - 1) / cmax, npass_total, -1};
   // launches alternate buffers: src -> X -> Y -> ... ; the last must be dst_final.
   // With src == dst_final an even launch count ends in dst_final; with src ==
   // other an odd count does.  Split a chunk to fix the parity, or copy.
@@ -2511,15 +2526,10 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
   // is even, dst_final if odd -- so any count works when src is foreign.
   const bool foreign = (src != dst_final && src != other);
   if (!foreign) {
-    const bool even = (chunks.size() % 2) == 0;
+    const bool even = (chunks.count() % 2) == 0;
     if (need_even != even) {
-      int k = -1;
-      for (size_t q = 0; q < chunks.size(); ++q)
-        if (chunks[q] >= 2) { k = (int)q; break; }
-      if (k >= 0) {
-        const int c = chunks[k];
-        chunks[k] = c / 2;
-        chunks.insert(chunks.begin() + k + 1, c - c / 2);
+      if (chunks.base(0) >= 2) {  // the largest chunk comes first
+        chunks.split = 0;
       } else {
         // single 1-pass chunk with the wrong parity: copy first
         T* to = src_is_final ? other : dst_final;
@@ -2529,11 +2539,12 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
       }
     }
   }
-  const int L = (int)chunks.size();
+  const int L = chunks.count();
   // destination of launch q: counted back from the last, which is dst_final
   const T* in = src;
   int colour = first_colour;
   for (int q = 0; q < L; ++q) {
+    const int npass = chunks.at(q);
     T* out = (((L - 1 - q) % 2) == 0) ? dst_final : other;
     SmoothArgs<T> a{};
     a.p_in = in;
@@ -2544,15 +2555,15 @@ int run_passes(int mode, T* dst_final, T* other, const T* src, const T* rhs,
     a.W = W;
     a.alpha = alpha;
     a.omega = omega;
-    a.npass = chunks[q];
+    a.npass = npass;
     a.first_colour = colour;
-    a.R = chunks[q];
+    a.R = npass;
     a.zero_init = (zero_first && q == 0) ? 1 : 0;
     a.coarse_div = 1.0;
     a.gate = gate;
     int rc = launch_tiled<T>(mode, a, s, true);
     if (rc) return rc;
-    colour = (colour + chunks[q]) & 1;
+    colour = (colour + npass) & 1;
     in = out;
   }
   return SMK_OK;
@@ -3528,7 +3539,76 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
 
 using namespace smk;
 
+// ------------------------------------------------- no-allocation Jacobi solve
+// Scratch of smk_jacobi_solve_noalloc: the cell flags (1 B per cell) and the
+// ping-pong partner of p (8 B per cell in float64) of a grid of at most
+// SMK_NOALLOC_MAX_CELLS cells, 9 MiB in all, resident on every GPU the
+// library runs on until the process ends.  Host code never passes the name
+// `scratch` to a kernel (on the host it is a stand-in, not a device
+// address): it uses the address hipGetSymbolAddress gives for the current
+// device, looked up once per device.
+constexpr size_t NOALLOC_FLAGS_BYTES = align256((size_t)SMK_NOALLOC_MAX_CELLS);
+constexpr size_t NOALLOC_BYTES = NOALLOC_FLAGS_BYTES + align256((size_t)SMK_NOALLOC_MAX_CELLS * 8);
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::atomic<unsigned char*> g_scratch[NOALLOC_MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch` (looked up on the device's first
+// call, then reused).
+int noalloc_scratch(unsigned char** out) {
+  int dev = -1;
+  SMK_CHECK_HIP(hipGetDevice(&dev));
+  SMK_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "no-allocation scratch: device %d", dev);
+  unsigned char* p = g_scratch[dev].load(std::memory_order_acquire);
+  if (p == nullptr) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch[dev].load(std::memory_order_relaxed);
+    if (p == nullptr) {
+      void* a = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || a == nullptr)
+        return fail(e != hipSuccess ? (int)e : (int)hipErrorUnknown,
+                    "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+      p = static_cast<unsigned char*>(a);
+      g_scratch[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return SMK_OK;
+}
+}  // namespace
+
 extern "C" {
+
+int smk_jacobi_solve_noalloc(void* p, const void* div, const uint8_t* fmu, const uint8_t* fmv,
+                             int H, int W, double rho, double h, double dt, int iterations,
+                             int dtype, void* stream) {
+  SMK_REQUIRE(p && div && fmu && fmv, "smk_jacobi_solve_noalloc: null pointer");
+  SMK_REQUIRE(H > 0 && W > 0 && iterations >= 0, "smk_jacobi_solve_noalloc: bad arguments");
+  SMK_REQUIRE(dtype == SMK_F64 || dtype == SMK_F32, "smk_jacobi_solve_noalloc: bad dtype %d",
+              dtype);
+  SMK_REQUIRE((long)H * W <= SMK_NOALLOC_MAX_CELLS,
+              "smk_jacobi_solve_noalloc: %d x %d = %ld cells exceeds the static scratch of %ld "
+              "cells; use smk_jacobi_solve with a workspace",
+              H, W, (long)H * W, (long)SMK_NOALLOC_MAX_CELLS);
+  unsigned char* base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  hipStream_t s = as_stream(stream);
+  uint8_t* flags = base;
+  void* ptmp = base + NOALLOC_FLAGS_BYTES;
+  rc = smk_cell_flags(fmu, fmv, H, W, flags, stream);
+  if (rc) return rc;
+  const double alpha = calc_alpha(h, rho, dt);
+  if (dtype == SMK_F64)
+    return run_passes<double>(MODE_JACOBI, (double*)p, (double*)ptmp, (const double*)p,
+                              (const double*)div, flags, H, W, alpha, 1.0, iterations, 0, s);
+  return run_passes<float>(MODE_JACOBI, (float*)p, (float*)ptmp, (const float*)p,
+                           (const float*)div, flags, H, W, alpha, 1.0, iterations, 0, s);
+}
 
 int smk_profile_enable(int on) {
   g_prof_on = on != 0;
