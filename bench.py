@@ -21,6 +21,16 @@ uneven scenes stepped in one batch, FP32 V-cycles in FP64 refinement to
 adjoint time, FP32 and FP64, and a batch of seeded targets); ``--only``
 runs one of them alone.
 
+Roofline: ``roofline`` puts the whole step against the HBM roofline (the
+north-star target): algorithmic bytes per step (one compulsory read or
+write per field per logical pass, ``algorithmic_bytes_per_cell``) over the
+HIP-event time of a step, as a fraction of 8 TB/s, with the step's HBM bytes
+measured by counters as ``traffic`` (profiles/traffic.json, written by
+``--traffic-from`` from ``--pmc-run`` counter runs).  ``roofline.kernel``
+names the family carrying most algorithmic bytes with its launch time and
+its counter-measured HBM rate as a fraction of the peak; its algorithmic
+rate counts each of a multi-pass launch's passes and is not a fraction.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -126,11 +136,16 @@ TRAFFIC_FAMILIES = [
     ("k_div_residual<double", "div_residual_f64")]
 
 
-def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30) -> dict:
+def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30,
+                     variant: str | None = None, steps: int = 0) -> dict:
     """HBM bytes per launch by kernel family from two rocprofv3 --pmc runs of
     this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch).  Each counter
     is scaled by what it reads for the run's own 1 GiB copy (calibrate_copy:
-    copy_bytes read and written), so the table is in true bytes."""
+    copy_bytes read and written), so the table is in true bytes.  With
+    ``variant`` and ``steps`` (a ``--pmc-run`` of that variant, which runs
+    ``steps`` substeps and nothing else on the GPU but the copy), the bytes of
+    every library kernel over the run divided by ``steps`` are recorded as
+    that variant's HBM bytes per step."""
     import collections
     import csv
 
@@ -140,9 +155,11 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30) 
                 return fam
         return None
 
-    per, factor = {}, {}
+    per, factor, lib_total = {}, {}, {}
     for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
         rows = list(csv.DictReader(open(path)))
+        lib_total[counter] = sum(float(r["Counter_Value"]) for r in rows
+                                 if "smk::k_" in r["Kernel_Name"])
         agg = collections.defaultdict(lambda: [0, 0.0])
         for r in rows:
             f = family(r["Kernel_Name"])
@@ -155,6 +172,12 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30) 
         factor[counter] = (copy_bytes / 1024) / (sum(big) / len(big))
     out = {"calibration": {"copy_bytes_each_way": copy_bytes, "fetch_factor": factor["FETCH_SIZE"],
                            "write_factor": factor["WRITE_SIZE"]}}
+    if variant and steps > 0:
+        out["per_step"] = {variant: {
+            "steps": steps,
+            "hbm_bytes_per_step": round((lib_total["FETCH_SIZE"] * factor["FETCH_SIZE"]
+                                         + lib_total["WRITE_SIZE"] * factor["WRITE_SIZE"])
+                                        * 1024 / steps)}}
     for f, (n, fk) in sorted(per["FETCH_SIZE"].items()):
         wk = per["WRITE_SIZE"].get(f, (0, 0.0))[1]
         out[f] = {"dispatches": n, "fetch_KiB_per_launch": round(fk, 1),
@@ -199,6 +222,7 @@ def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3
         "ms_per_step": ms / steps,
         "wall_ms_per_step": wall * 1e3 / steps,
         "cell_steps_per_s": scene.cells * steps / (ms / 1e3),
+        "cells": scene.cells,
         "v_cycles_per_step": sum(cycles) / steps,
         "refinements_per_step": sum(refinements) / steps,
         "rel_residual_last": sim.last_solve_info.get("rel_residual"),
@@ -206,10 +230,79 @@ def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3
     }, sim
 
 
+def step_roofline(r: dict, tmap: dict, var: str) -> dict:
+    """The whole step against the HBM roofline: algorithmic bytes per step
+    (algorithmic_bytes_per_cell x cells) over the HIP-event time of a step;
+    ``traffic`` is the step's HBM bytes measured by counters (FETCH_SIZE +
+    WRITE_SIZE of every library kernel over a --pmc-run of this variant,
+    profiles/traffic.json), null without such a run.  ``kernel``: the family
+    carrying most algorithmic bytes, with its HIP-event launch time, its
+    algorithmic rate (a multi-pass kernel counts every pass, so this rate can
+    exceed the HBM peak and is not a fraction of it) and its counter-measured
+    HBM bytes and rate, the latter as a fraction of the peak."""
+    alg_step = r["alg_bytes_per_cell_step"] * r["cells"]
+    achieved = alg_step / (r["ms_per_step"] * 1e6)
+    per_step = tmap.get("per_step", {}).get(var, {}).get("hbm_bytes_per_step")
+    out = {"bound": "hbm", "scope": "whole step", "achieved": achieved, "peak": PEAK_HBM_GBS,
+           "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": per_step,
+           "alg_bytes_per_step": alg_step, "ms_per_step": r["ms_per_step"],
+           "traffic_GBps": None if per_step is None else per_step / (r["ms_per_step"] * 1e6),
+           "traffic_source": tmap.get("source") if per_step is not None else None}
+    if r["kernels"]:
+        k = max(r["kernels"], key=lambda q: q["alg_bytes"])
+        hbm = tmap.get(k["name"], {}).get("hbm_bytes_per_launch")
+        kg = None if hbm is None else hbm / (k["avg_launch_us"] * 1e3)
+        out["kernel"] = {"name": k["name"], "avg_launch_us": k["avg_launch_us"],
+                         "alg_bytes_per_launch": k["alg_bytes_per_launch"],
+                         "alg_GBps": k["alg_GBps"], "hbm_bytes_per_launch": hbm,
+                         "hbm_GBps": kg, "hbm_frac": None if kg is None else kg / PEAK_HBM_GBS}
+    return out
+
+
+def mixed_vs_fp64(sims: dict, scene) -> dict:
+    """Relative L2 difference of one mixed-variant step from one FP64 step
+    taken from the same state (the FP64 run's fields after its timed and
+    profiled steps, copied into the mixed simulation); pressure modulo its
+    mean (the all-fluid Neumann problem fixes it up to a constant).  Rollouts
+    of the two variants drift apart over many steps: at this workload's CFL
+    numbers the flow amplifies any difference at the solver's tolerance, as
+    it does between FP64 runs at two tolerances."""
+    import torch
+
+    if "fp64" not in sims or "mixed" not in sims:
+        return {}
+    a, b = sims["mixed"], sims["fp64"]
+    for f in ("density", "velocity_u", "velocity_v"):
+        getattr(a, f).copy_(getattr(b, f))
+    a.substep(scene.dt)
+    b.substep(scene.dt)
+    torch.cuda.synchronize()
+    out = {"mixed_rel_residual": a.last_solve_info.get("rel_residual"),
+           "fp64_rel_residual": b.last_solve_info.get("rel_residual")}
+    for f in ("density", "velocity_u", "velocity_v", "pressure"):
+        x, y = getattr(a, f), getattr(b, f)
+        if f == "pressure":
+            x, y = x - x.mean(), y - y.mean()
+        out[f] = float((x - y).norm() / y.norm().clamp_min(1e-300))
+    return out
+
+
+def _cpu_model() -> str | None:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(scene, sample_steps: int):
     import oracle
     import oracle.sim
 
+    cpus = len(os.sched_getaffinity(0))  # before the OpenMP runtime binds this thread
     ref = oracle.sim.from_scene(scene)
     ref.substep(scene.dt)  # warm-up (page-in, thread start)
     t0 = time.perf_counter()
@@ -219,7 +312,9 @@ def cpu_baseline(scene, sample_steps: int):
         cyc += ref.last_solve["iterations"]
     dt = time.perf_counter() - t0
     return {"value": scene.cells * sample_steps / dt, "unit": "cell-steps/s",
-            "cores": oracle.num_threads(), "kind": "port",
+            "cores": oracle.num_threads(), "kind": "port", "cpu_model": _cpu_model(),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "cpus_available": cpus,
             "sample": (f"{sample_steps} steps of the same {scene.H}x{scene.W} workload after 1 "
                        f"warm-up step, FP64 oracle (C/OpenMP restatement), "
                        f"{cyc / sample_steps:.1f} V-cycles/step, {dt:.1f} s")}
@@ -461,19 +556,41 @@ def main():
     ap.add_argument("--only", choices=("batched", "diff", "small"), default=None,
                     help="run one secondary workload alone and print its JSON (profiling runs)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="write profiles/traffic.json from the counter_collection.csv files of "
-                         "two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this script, "
-                         "then exit")
+                    help="merge into profiles/traffic.json the HBM bytes from the "
+                         "counter_collection.csv files of two rocprofv3 --pmc runs (FETCH_SIZE, "
+                         "WRITE_SIZE) of `bench.py --pmc-run VARIANT`, then exit")
+    ap.add_argument("--traffic-variant", default=None,
+                    help="with --traffic-from: the variant the counter runs ran")
+    ap.add_argument("--pmc-run", choices=("fp64", "mixed"), default=None,
+                    help="counter run: only this variant's warm-up and timed steps and the "
+                         "calibration copy, no profiling, other workloads or CPU baseline; "
+                         "prints the step count --traffic-from needs")
     args = ap.parse_args()
 
     if args.traffic_from:
-        table = traffic_from_pmc(*args.traffic_from)
-        with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        tfile = os.path.join(ROOT, "profiles", "traffic.json")
+        table = {}
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                table = json.load(f)
+        steps = args.warmup + args.steps if args.traffic_variant else 0
+        new = traffic_from_pmc(*args.traffic_from, variant=args.traffic_variant, steps=steps)
+        per_step = dict(table.get("per_step", {}))
+        per_step.update(new.pop("per_step", {}))
+        table.update(new)
+        table["per_step"] = per_step
+        table["source"] = ("rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of `python3 bench.py "
+                           "--pmc-run VARIANT --steps K --warmup W`; per_step = all library "
+                           "kernels of the run / (W + K)")
+        with open(tfile, "w") as f:
             json.dump(table, f, indent=1)
             f.write("\n")
         print(json.dumps(table, indent=1))
         return
 
+    # the CPU baseline's OpenMP threads stay on the cores they start on; set
+    # before the oracle's OpenMP runtime is loaded (its first call)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(it measures one GPU; nothing here runs across GPUs)", file=sys.stderr)
@@ -499,14 +616,23 @@ def main():
 
     log(f"generating plume_{args.N}_mg")
     scene = scenes.plume_2048_mg(N=args.N)
-    results = {}
+    if args.pmc_run:
+        var = args.pmc_run
+        run_variant(scene, "mixed" if var == "mixed" else "native", args.steps, args.warmup,
+                    profile=0, inner=args.mixed_inner if var == "mixed" else 1,
+                    coarsest=args.mixed_coarsest if var == "mixed" else "sweeps",
+                    arith=args.mixed_arith if var == "mixed" else "float64")
+        calibrate_copy()
+        print(json.dumps({"pmc_run": var, "steps": args.warmup + args.steps}), flush=True)
+        return
+    results, sims = {}, {}
     for var in args.variants.split(","):
         log(f"variant {var}: warmup {args.warmup}, steps {args.steps}")
         inner = args.mixed_inner if var == "mixed" else 1
         coarsest = args.mixed_coarsest if var == "mixed" else "sweeps"
         arith = args.mixed_arith if var == "mixed" else "float64"
-        r, _ = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
-                           args.warmup, inner=inner, coarsest=coarsest, arith=arith)
+        r, sims[var] = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
+                                   args.warmup, inner=inner, coarsest=coarsest, arith=arith)
         r["inner_cycles_per_refinement"] = inner
         r["coarsest_solve"] = coarsest
         r["inner_arithmetic"] = arith if var == "mixed" else None
@@ -519,33 +645,15 @@ def main():
     # the headline variant is the faster one; both are reported
     main_var = min(results, key=lambda k: results[k]["ms_per_step"])
     m = results[main_var]
+    accuracy = mixed_vs_fp64(sims, scene)
+    sims.clear()
     copy_gbps = calibrate_copy()
-    # the roofline kernel: the family carrying most of the step's algorithmic
-    # HBM bytes (for an HBM roofline the kernel that sets the step's traffic;
-    # its HIP-event time matches rocprof's kernel time within a few percent,
-    # while event pairs around short coarse-level launches add dispatch time)
-    dom = max(m["kernels"], key=lambda k: k["alg_bytes"]) if m["kernels"] else None
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     tmap = {}
     if os.path.exists(tfile):
         with open(tfile) as f:
             tmap = json.load(f)
-
-    def traffic_of(name):
-        return tmap.get(name, {}).get("hbm_bytes_per_launch")
-
-    def roofline_of(k):
-        return {"bound": "hbm", "kernel": k["name"], "achieved": k["alg_GBps"],
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": k["alg_GBps"] / PEAK_HBM_GBS,
-                "traffic": traffic_of(k["name"]), "avg_launch_us": k["avg_launch_us"],
-                "alg_bytes_per_launch": k["alg_bytes_per_launch"]}
-
-    traffic = traffic_of(dom["name"]) if dom else None
-    fine = {}
-    for var, r in results.items():
-        for k in r["kernels"]:
-            if k["name"].startswith("rb_smooth_fine"):
-                fine[var] = roofline_of(k)
+    roof = {var: step_roofline(r, tmap, var) for var, r in results.items()}
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_steps} oracle steps")
@@ -586,12 +694,12 @@ def main():
                    "coarsest_solve": results[main_var].get("coarsest_solve", "sweeps"),
                    "inner_arithmetic": results[main_var].get("inner_arithmetic")},
         "variants": results,
-        "roofline": None if dom is None else dict(
-            roofline_of(dom), whole_step_alg_GBps=m["alg_GBps_whole_step"],
-            copy_kernel_GBps=copy_gbps),
-        "fine_level_roofline": fine,
+        "roofline": dict(roof[main_var], copy_kernel_GBps=copy_gbps),
+        "roofline_by_variant": roof,
         "cpu_baseline": cpu,
     }
+    if accuracy:
+        line["mixed_vs_fp64_rel_err"] = accuracy
     if "fp64" in results and "mixed" in results:
         line["mixed_speedup_vs_fp64"] = results["fp64"]["ms_per_step"] / results["mixed"]["ms_per_step"]
     if "differentiable" in extra:
