@@ -188,10 +188,15 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30,
 
 
 def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3,
-                inner: int = 1, coarsest: str = "sweeps", arith: str = "float64"):
+                inner: int = 1, coarsest: str = "sweeps", arith: str = "float64",
+                tune: bool = True, graphs: bool = True):
+    """Warm-up steps, the autotuner's choice of tile variants (same results,
+    timed on this simulation from a saved state), ``steps`` timed steps,
+    then as many steps with the stages around the solve replayed from HIP
+    graphs, and the profiled steps."""
     import torch
 
-    from cfd_amd import scenes
+    from cfd_amd import scenes, tuning
 
     sim = scenes.make_simulation(scene, dtype="float64", device="cuda",
                                  solver_overrides={"precision": precision, "inner_cycles": inner,
@@ -200,6 +205,7 @@ def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3
     torch.cuda.synchronize()
     for _ in range(warmup):
         sim.substep(scene.dt)
+    tuned = tuning.autotune(sim, scene.dt, steps=2, repeats=2) if tune else None
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     start = torch.cuda.Event(enable_timing=True)
@@ -216,11 +222,26 @@ def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = start.elapsed_time(end)
+    graph_ms = None
+    if graphs:
+        sim.use_graphs = True
+        for _ in range(2):  # captures both buffer assignments
+            sim.substep(scene.dt)
+        torch.cuda.synchronize()
+        start.record(stream)
+        for _ in range(steps):
+            sim.substep(scene.dt)
+        end.record(stream)
+        torch.cuda.synchronize()
+        graph_ms = start.elapsed_time(end) / steps
+        sim.use_graphs = False
     kernels = profile_steps(sim, scene, profile) if profile > 0 else []
     return {
         "kernels": kernels,
         "ms_per_step": ms / steps,
         "wall_ms_per_step": wall * 1e3 / steps,
+        "graph_ms_per_step": graph_ms,
+        "autotune": None if tuned is None else tuned["chosen"],
         "cell_steps_per_s": scene.cells * steps / (ms / 1e3),
         "cells": scene.cells,
         "v_cycles_per_step": sum(cycles) / steps,
@@ -296,6 +317,20 @@ def _cpu_model() -> str | None:
     except OSError:
         pass
     return None
+
+
+def cpu_baseline_in_child(N: int, sample_steps: int) -> dict:
+    """cpu_baseline in a child process started with OMP_PROC_BIND=close: the
+    OpenMP threads stay on the cores they start on, and this process (whose
+    torch runtime would pin its main thread under that setting) is not
+    bound."""
+    import subprocess
+
+    env = dict(os.environ, OMP_PROC_BIND="close")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+                        str(N), str(sample_steps)], env=env, stdout=subprocess.PIPE, text=True,
+                       check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def cpu_baseline(scene, sample_steps: int):
@@ -561,6 +596,8 @@ def main():
                          "WRITE_SIZE) of `bench.py --pmc-run VARIANT`, then exit")
     ap.add_argument("--traffic-variant", default=None,
                     help="with --traffic-from: the variant the counter runs ran")
+    ap.add_argument("--cpu-baseline-child", nargs=2, type=int, metavar=("N", "STEPS"),
+                    help=argparse.SUPPRESS)
     ap.add_argument("--pmc-run", choices=("fp64", "mixed"), default=None,
                     help="counter run: only this variant's warm-up and timed steps and the "
                          "calibration copy, no profiling, other workloads or CPU baseline; "
@@ -588,9 +625,13 @@ def main():
         print(json.dumps(table, indent=1))
         return
 
-    # the CPU baseline's OpenMP threads stay on the cores they start on; set
-    # before the oracle's OpenMP runtime is loaded (its first call)
-    os.environ.setdefault("OMP_PROC_BIND", "close")
+    if args.cpu_baseline_child:
+        # the CPU baseline, alone in a child process that never touches the GPU
+        n, k = args.cpu_baseline_child
+        from cfd_amd import scenes
+
+        print(json.dumps(cpu_baseline(scenes.plume_2048_mg(N=n), k)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(it measures one GPU; nothing here runs across GPUs)", file=sys.stderr)
@@ -619,7 +660,8 @@ def main():
     if args.pmc_run:
         var = args.pmc_run
         run_variant(scene, "mixed" if var == "mixed" else "native", args.steps, args.warmup,
-                    profile=0, inner=args.mixed_inner if var == "mixed" else 1,
+                    profile=0, tune=False, graphs=False,
+                    inner=args.mixed_inner if var == "mixed" else 1,
                     coarsest=args.mixed_coarsest if var == "mixed" else "sweeps",
                     arith=args.mixed_arith if var == "mixed" else "float64")
         calibrate_copy()
@@ -657,7 +699,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_steps} oracle steps")
-        cpu = cpu_baseline(scene, args.cpu_steps)
+        cpu = cpu_baseline_in_child(args.N, args.cpu_steps)
         log(f"cpu baseline: {cpu}")
 
     extra = {}

@@ -47,6 +47,13 @@ class Emitter2D:
     def writes_temperature(self) -> bool:
         return self.enabled and self.temperature_value != 0.0
 
+    def graph_state(self):
+        """Everything emit()'s launches depend on besides the fields and dt
+        (a captured step stage is replayed only while this is unchanged)."""
+        return ("emitter2d", self.enabled, tuple(map(float, self.position)), float(self.radius),
+                self.approach, float(self.density_value), float(self.temperature_value),
+                tuple(map(float, self.velocity_value)))
+
     def set_enabled(self, enabled):
         self.enabled = enabled
 
@@ -95,6 +102,10 @@ class BoxEmitter:
     @classmethod
     def from_box(cls, box, mode="set"):
         return cls(box.x0, box.y0, box.x1, box.y1, box.value, mode)
+
+    def graph_state(self):
+        """What emit()'s launch depends on besides the field (Emitter2D.graph_state)."""
+        return ("box", self.enabled, self.x0, self.y0, self.x1, self.y1, self.value, self.mode)
 
     def emit(self, density_field, velocity_u_field, velocity_v_field, temperature_field, dt):
         if self.enabled and density_field is not None:

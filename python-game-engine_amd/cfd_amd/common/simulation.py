@@ -127,6 +127,10 @@ class SmokeSimulation2D:
         self._watch_temperature()
         self.compute_metrics = True
         self.last_solve_info: dict = {}
+        # HIP graphs of the step's stages before and after the pressure solve
+        # (see _run_stage); off by default
+        self.use_graphs = False
+        self._graphs: dict = {}
 
     def _init_boundaries(self):
         """simulation.py:126-131."""
@@ -204,37 +208,68 @@ class SmokeSimulation2D:
             self._mask_dirty = False
 
         t = prof.timing
+        graphs = self._graphs_usable()
         with t.scope("simulation.substep"):
-            with t.scope("simulation.emitters"):
-                self._apply_emitters(dt)
-            with t.scope("simulation.forces"):
-                self._apply_forces(dt)
-            with t.scope("simulation.advection.velocity"):
-                self._advect_velocity(dt)
-            fused = self._all_hip()
-            with t.scope("simulation.advection.density"):
-                self._advect_density(dt, fuse_dissipation=fused)
-            temp = self._temperature_active()
-            if temp:
-                with t.scope("simulation.advection.temperature"):
-                    self._advect_temperature(dt)
-                with t.scope("simulation.cooling"):
-                    self._cool_temperature(dt)
-            if not fused:
-                with t.scope("simulation.dissipation"):
-                    hip.scale_field(self.density, self.density_dissipation)
-            with t.scope("simulation.divergence"):
-                self._compute_divergence()
-            with t.scope("profiling.capture"):
-                prof.metrics.capture("divergence", "before_projection", self.divergence)
+            if graphs:
+                self._run_stage("pre", dt, self._pre_solve)
+            else:
+                self._pre_solve(dt)
             with t.scope("simulation.pressure.solve"):
                 self._solve_pressure()
-            with t.scope("simulation.pressure.apply"):
-                self._apply_pressure_gradient(dt, fuse_boundaries=fused)
-            if not fused:
-                with t.scope("simulation.boundaries"):
-                    self._enforce_boundaries()
+            if not graphs:
+                self._project(dt)
+        if graphs:  # projection and metrics pass in one graph (profiling is off)
+            self._run_stage("post", dt, self._post_solve)
+        else:
+            self._metrics(dt)
 
+    def _pre_solve(self, dt):
+        """simulation.py:205-237: sources, forces, transport, divergence."""
+        prof = self.profiler
+        t = prof.timing
+        with t.scope("simulation.emitters"):
+            self._apply_emitters(dt)
+        with t.scope("simulation.forces"):
+            self._apply_forces(dt)
+        with t.scope("simulation.advection.velocity"):
+            self._advect_velocity(dt)
+        fused = self._all_hip()
+        with t.scope("simulation.advection.density"):
+            self._advect_density(dt, fuse_dissipation=fused)
+        temp = self._temperature_active()
+        if temp:
+            with t.scope("simulation.advection.temperature"):
+                self._advect_temperature(dt)
+            with t.scope("simulation.cooling"):
+                self._cool_temperature(dt)
+        if not fused:
+            with t.scope("simulation.dissipation"):
+                hip.scale_field(self.density, self.density_dissipation)
+        with t.scope("simulation.divergence"):
+            self._compute_divergence()
+        with t.scope("profiling.capture"):
+            prof.metrics.capture("divergence", "before_projection", self.divergence)
+
+    def _post_solve(self, dt):
+        """simulation.py:242-265: projection, walls, the metrics pass."""
+        self._project(dt)
+        self._metrics(dt)
+
+    def _project(self, dt):
+        """simulation.py:242-247."""
+        t = self.profiler.timing
+        fused = self._all_hip()
+        with t.scope("simulation.pressure.apply"):
+            self._apply_pressure_gradient(dt, fuse_boundaries=fused)
+        if not fused:
+            with t.scope("simulation.boundaries"):
+                self._enforce_boundaries()
+
+    def _metrics(self, dt):
+        """simulation.py:250-265 (outside the substep's timing scope)."""
+        prof = self.profiler
+        t = prof.timing
+        fused = self._all_hip()
         if self.compute_metrics:
             with t.scope("profiling.metrics"):
                 alpha = calculate_alpha(self.cell_size, self.fluid_density, dt)
@@ -255,6 +290,50 @@ class SmokeSimulation2D:
                                      backend=self.operators_backend.residual)
                 prof.metrics.capture("pressure_residual", "final", self.pressure_residual)
                 prof.metrics.process()
+
+    # ------------------------------------------------------------- graphs
+    _BUFFERS = ("density", "density_temp", "velocity_u", "velocity_u_temp", "velocity_v",
+                "velocity_v_temp", "temperature", "temperature_temp")
+
+    def _graphs_usable(self) -> bool:
+        """Graphs replace the eager stages when asked for, every operator
+        runs on the hip backend (a fixed sequence of library kernels), no
+        profiling is on (its scopes record events and read back) and every
+        emitter can state what its launch depends on (graph_state)."""
+        return (self.use_graphs and self.device.type == "cuda" and self._all_hip()
+                and not self.profiler.enabled
+                and all(hasattr(e, "graph_state") for e in self.emitters))
+
+    def invalidate_graphs(self):
+        """Drop the captured stages (they are captured again on next use).
+        Changes the stage keys cover need no call: dt, the buffer
+        assignment, the temperature passes, the emitters' states, the
+        solid-mask tensor and the metrics switch."""
+        self._graphs.clear()
+
+    def _run_stage(self, stage, dt, fn):
+        """Run a stage from a HIP graph of its kernels.  A stage's kernels
+        and their arguments are fixed by the key below; the ping-pong swaps
+        of the transported fields (simulation.py:285-298) change the buffer
+        assignment every step, so each assignment has its graph, captured on
+        first use (capture runs the stage's Python, including its swaps, and
+        records the kernels without running them, so the graph is replayed
+        once at once) and afterwards replayed with the swaps applied in
+        Python."""
+        key = (stage, float(dt), self._temperature_active(), self.compute_metrics,
+               self._solid_mask.data_ptr(), tuple(e.graph_state() for e in self.emitters),
+               tuple(getattr(self, b).data_ptr() for b in self._BUFFERS))
+        entry = self._graphs.get(key)
+        if entry is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                fn(dt)
+            entry = (g, {b: getattr(self, b) for b in self._BUFFERS})
+            self._graphs[key] = entry
+        g, after = entry
+        g.replay()
+        for b, tensor in after.items():
+            setattr(self, b, tensor)
 
     # --------------------------------------------------------- components
     def _apply_emitters(self, dt):
