@@ -165,8 +165,11 @@ struct smk_batch {
   int* d_cycles = nullptr;
   int* d_count = nullptr;
   int* d_parity = nullptr;
-  int* h_count = nullptr;  // pinned ring, one slot per queued iteration
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // pinned ring of active counts, one slot per queued iteration (4 slots),
+  // and its device address; ring_seq numbers the publications
+  smk::CountSlot* h_ring = nullptr;
+  smk::CountSlot* d_ring = nullptr;
+  unsigned ring_seq = 1;
   // V-cycle
   std::vector<smk::BatchDepth> depth;
   smk::SmoothArgs<float>* d_args = nullptr;
@@ -425,6 +428,16 @@ static int batch_build_graphs(smk_batch* b) {
   return rc;
 }
 
+// The active count of one batch iteration into a slot of the pinned host
+// ring, then the slot's sequence number (after a system-scope fence), so the
+// host can poll the slot.
+__global__ void k_publish_count(const int* __restrict__ count, CountSlot* slot, unsigned seq) {
+  if (threadIdx.x != 0) return;
+  slot->count = *count;
+  __threadfence_system();
+  slot->seq = seq;
+}
+
 static void batch_release(smk_batch* b) {
   if (!b) return;
   batch_free_graphs(b);
@@ -433,9 +446,7 @@ static void batch_release(smk_batch* b) {
                   (void*)b->d_b2, (void*)b->d_rel, (void*)b->d_active, (void*)b->d_cycles,
                   (void*)b->d_count, (void*)b->d_parity})
     if (p) (void)hipFree(p);
-  if (b->h_count) (void)hipHostFree(b->h_count);
-  for (hipEvent_t e : b->ev)
-    if (e) (void)hipEventDestroy(e);
+  if (b->h_ring) (void)hipHostFree(b->h_ring);
   for (smk_mg* m : b->mg)
     if (m) smk_mg_destroy(m);
   if (b->cap) (void)hipStreamDestroy(b->cap);
@@ -542,9 +553,11 @@ int smk_batch_create(const smk_batch_scene* scenes, int n, const smk_batch_param
   if (!rc) rc = mg_alloc((void**)&b->d_count, sizeof(int));
   if (!rc) rc = mg_alloc((void**)&b->d_parity, sizeof(int));
   if (!rc) {
-    hipError_t e = hipHostMalloc((void**)&b->h_count, 4 * sizeof(int), hipHostMallocDefault);
-    for (int k = 0; k < 4 && e == hipSuccess; ++k)
-      e = hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming);
+    hipError_t e = hipHostMalloc((void**)&b->h_ring, 4 * sizeof(CountSlot), hipHostMallocDefault);
+    if (e == hipSuccess) {
+      memset(b->h_ring, 0, 4 * sizeof(CountSlot));
+      e = hipHostGetDevicePointer((void**)&b->d_ring, b->h_ring, 0);
+    }
     if (e != hipSuccess) rc = fail((int)e, "smk_batch_create: %s", hipGetErrorString(e));
   }
   if (!rc) {
@@ -581,7 +594,13 @@ int smk_batch_step(smk_batch* b, smk_batch_info* info, void* stream) {
   }
   // the active count of iteration k-1 is read while iteration k is queued
   // (an iteration after the last useful one is a no-op: every launch skips
-  // inactive scenes)
+  // inactive scenes).  Each iteration ends with k_publish_count, which
+  // writes the count and then its sequence number into a slot of the pinned
+  // ring; the host polls the slot of iteration k-1 -- no copy, event or
+  // wait on the stream per iteration.
+  volatile CountSlot* ring = b->h_ring;
+  const unsigned base = b->ring_seq;
+  b->ring_seq = base + (unsigned)b->prm.max_cycles + 4u;
   int it = 0, done_at = -1;
   for (; it <= b->prm.max_cycles + 1; ++it) {
     if (graphs) {
@@ -590,14 +609,24 @@ int smk_batch_step(smk_batch* b, smk_batch_info* info, void* stream) {
       rc = batch_stage_iter(b, s);
       if (rc) return rc;
     }
-    const int slot = it & 3;
-    SMK_CHECK_HIP(hipMemcpyAsync(&b->h_count[slot], b->d_count, sizeof(int),
-                                 hipMemcpyDeviceToHost, s));
-    SMK_CHECK_HIP(hipEventRecord(b->ev[slot], s));
+    hipLaunchKernelGGL(k_publish_count, dim3(1), dim3(64), 0, s, b->d_count,
+                       b->d_ring + (it & 3), base + (unsigned)it);
+    rc = check_launch("k_publish_count");
+    if (rc) return rc;
     if (it >= 1) {
       const int prev = (it - 1) & 3;
-      SMK_CHECK_HIP(hipEventSynchronize(b->ev[prev]));
-      if (b->h_count[prev] == 0) {
+      const unsigned want = base + (unsigned)(it - 1);
+      for (unsigned spin = 1; ring[prev].seq != want; ++spin) {
+        if ((spin & 1023u) == 0u) {
+          const hipError_t e = hipStreamQuery(s);
+          if (e == hipSuccess && ring[prev].seq != want)
+            return fail((int)hipErrorUnknown, "batch step: count %u never published", want);
+          if (e != hipSuccess && e != hipErrorNotReady)
+            return fail((int)e, "batch step: %s", hipGetErrorString(e));
+        }
+      }
+      __atomic_thread_fence(__ATOMIC_ACQUIRE);
+      if (ring[prev].count == 0) {
         done_at = it - 1;
         break;
       }

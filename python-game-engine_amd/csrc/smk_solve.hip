@@ -214,7 +214,10 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
         const uint8_t f = sf[o];
         T ce, ns, b;
         stencil_load(cur, sr, o, RW, f, ce, ns, b);
-        nv[k] = (T)relax_value(0, ce, ns, b, a.alpha, a.omega);
+        if (sizeof(T) == 4 && a.fast_math)
+          nv[k] = (T)relax_f((float)ce, (float)ns, (float)b, (float)a.alpha, (float)a.omega);
+        else
+          nv[k] = (T)relax_value(0, ce, ns, b, a.alpha, a.omega);
         oo[k] = o;
         st[k] = valid && (f & F_FLUID);
       }
@@ -277,7 +280,9 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
           const uint8_t f = sf[o];
           T ce, ns, b;
           stencil_load(cur, sr, o, RW, f, ce, ns, b);
-          const T rv = (T)residual_value(ce, ns, b, a.alpha);
+          const T rv = (sizeof(T) == 4 && a.fast_math && MODE == MODE_RB)
+                           ? (T)residual_f((float)ce, (float)ns, (float)b, (float)a.alpha)
+                           : (T)residual_value(ce, ns, b, a.alpha);
           const bool fl = f & F_LFLUID;
           total += fl ? (double)rv : 0.0;
           count += fl ? 1 : 0;
@@ -297,7 +302,9 @@ __device__ __forceinline__ void smooth_tile64_body(const SmoothArgs<T>& a, int b
       const uint8_t f = sf[o];
       T ce, ns, b;
       stencil_load(cur, sr, o, RW, f, ce, ns, b);
-      const double rv = residual_value(ce, ns, b, a.alpha);
+      const double rv = (sizeof(T) == 4 && a.fast_math && MODE == MODE_RB)
+                            ? (double)residual_f((float)ce, (float)ns, (float)b, (float)a.alpha)
+                            : residual_value(ce, ns, b, a.alpha);
       acc += (xout && gy < H && (f & F_LFLUID)) ? rv * rv : 0.0;
     }
     const double tot = block_sum<NT>(acc, red);
@@ -2023,6 +2030,13 @@ struct RingSlot {
   double rel;
   unsigned seq;
   unsigned pad;
+};
+
+// A slot of the batched scenes' pinned ring: the active scene count after
+// one iteration, then its sequence number.
+struct CountSlot {
+  int count;
+  unsigned seq;
 };
 
 __global__ void k_state_init(SolveState* __restrict__ st) {

@@ -246,8 +246,17 @@ def test_kink_slope_is_mean_of_one_sided_slopes(ops):
 
 def test_plume_256_gradient_central_differences():
     """The 256x256, 64-step rollout from zero velocity and unit strengths:
-    adjoint directional derivatives vs central differences (eps = 1e-6,
-    seeded directions, FP64, converged solves)."""
+    adjoint directional derivatives vs central differences (32 seeded
+    directions, FP64, converged solves) within 1e-6 relative.
+
+    At zero velocity every backtrace lands on a grid line, where the
+    bilinear interpolation has a kink: the adjoint takes the mean of the
+    one-sided slopes, and a central difference approaches that mean with an
+    error that shrinks with eps (measured over the same directions, max /
+    median: 3.2e-4 / 5.7e-6 at eps 1e-5, 3.0e-6 / 2.2e-7 at 1e-6, 3.2e-7 /
+    2.5e-8 at 1e-7, rounding still far below).  Hence eps = 1e-7, and the
+    error must also fall from eps = 1e-6 to 1e-7 (the difference converging
+    to the adjoint, not to another value)."""
     from cfd_amd import diff, scenes
     sc = scenes.diff_rollout_256()
     m = diff.from_scene(sc, tolerance=0.0, v_cycles=30)
@@ -259,17 +268,20 @@ def test_plume_256_gradient_central_differences():
     rho, _, _ = m.rollout(u0, v0, s, steps, checkpoint_every=8)
     diff.density_loss(rho, target).backward()
     rng = np.random.default_rng(50)
-    worst = 0.0
+    worst = {1e-6: 0.0, 1e-7: 0.0}
     for _ in range(32):
         du = cu(rng.standard_normal((N, N + 1)))
         dv = cu(rng.standard_normal((N + 1, N)))
         ds = cu(rng.standard_normal(steps))
         adj = float((u0.grad * du).sum() + (v0.grad * dv).sum() + (s.grad * ds).sum())
-        with torch.no_grad():
-            fd = diff.directional_fd(m, u0.detach(), v0.detach(), s.detach(), steps, target,
-                                     du, dv, ds, 1e-6)
-        worst = max(worst, abs(fd - adj) / abs(fd))
-    assert worst <= 1e-5, worst
+        for eps in worst:
+            with torch.no_grad():
+                fd = diff.directional_fd(m, u0.detach(), v0.detach(), s.detach(), steps,
+                                         target, du, dv, ds, eps)
+            worst[eps] = max(worst[eps], abs(fd - adj) / abs(fd))
+    print("gradient vs central differences, max relative error by eps:", worst)
+    assert worst[1e-7] <= 1e-6, worst
+    assert worst[1e-7] < 0.5 * worst[1e-6], worst
 
 
 def test_fp32_gradient_tracks_fp64():

@@ -158,3 +158,24 @@ def test_tile_variant_tolerance_solve(name):
     assert info["iterations"] == n_ref
     assert info["rel_residual"] == pytest.approx(rel_ref, rel=1e-9)
     np.testing.assert_array_equal(got, ref)
+
+
+# the variants whose passes honour float32 arithmetic (SmoothArgs::fast_math);
+# the register-resident 64x64 kernel (reg64) keeps the float64 expression and
+# is not offered for float32 inner arithmetic
+FAST_VARIANTS = [v for v in TILE_VARIANTS if v != "reg64"]
+
+
+@pytest.mark.parametrize("which", ["fine", "coarse_small"])
+@pytest.mark.parametrize("name", FAST_VARIANTS)
+def test_tile_variant_mixed_float32_arithmetic(which, name):
+    """FP32 V-cycles in float32 arithmetic inside FP64 refinement (the
+    headline configuration's solve): every variant the autotuner may pick
+    gives the default variant's pressure and cycle counts, bit for bit."""
+    kw = dict(v_cycles=60, tolerance=1e-9, precision="mixed", inner_cycles=2,
+              inner_arithmetic="float32", coarsest_solve="operator")
+    want, winfo = run_mg("float64", **kw)
+    with variant("float32", which, name):
+        got, info = run_mg("float64", **kw)
+    np.testing.assert_array_equal(got, want)
+    assert info["refinements"] == winfo["refinements"]
