@@ -11,9 +11,12 @@ V-cycles until ||r|| / ||alpha*div|| <= 1e-8.
 A step is one full SmokeSimulation2D.substep (sources, buoyancy, advection of
 u, v and density, divergence, pressure solve, projection + walls, metrics
 divergence and residual) with all fields resident in HBM.  Two variants:
-``fp64`` (FP64 V-cycles) and ``mixed`` (FP32 V-cycles inside FP64 defect
-correction, same FP64 tolerance).  ``value`` is the faster variant's
-cell-steps/s (``config.variant`` names it); both are reported.
+``fp64`` (the FP64 V-cycles above) and ``mixed`` (FP32 V-cycles inside FP64
+defect correction, same FP64 tolerance; by default their 16x16 level is
+solved exactly, by its precomputed least-squares solution operator, in place
+of the 50 sweeps, whose partial solve is what bounds the V-cycles'
+convergence).  ``value`` is the faster variant's cell-steps/s
+(``config.variant`` names it); both are reported.
 
 The same line carries the two secondary workloads: ``batched_scenes`` (96
 uneven scenes stepped in one batch, FP32 V-cycles in FP64 refinement to
@@ -55,17 +58,20 @@ def log(msg: str) -> None:
 
 
 def algorithmic_bytes_per_cell(v_cycles_per_step: float, mixed: bool, levels: int,
-                               refinements: float = 0.0) -> float:
+                               refinements: float = 0.0, coarsest_op: bool = False) -> float:
     """Bytes per cell-step by logical pass (float64 8 B, float32 4 B):
     non-solve passes 168, metrics divergence + residual 48, per FP64 V-cycle
     176.06 (V(2,2) smoothing 4x24, residual+restrict 18, prolong+correct 18,
     summed over levels x 4/3, plus 50 coarsest sweeps) + 16 for the norm
-    check; mixed: per FP32 V-cycle 88.03, per refinement 40."""
+    check; mixed: per FP32 V-cycle 88.03, per refinement 40.  With
+    ``coarsest_op`` the coarsest level is one read of its float32 256x256
+    operator (4 x 256 B per coarsest cell) in place of the sweeps."""
     geo = sum(0.25 ** l for l in range(levels - 1))
     coarse_cells = 0.25 ** (levels - 1)
-    per_vc64 = 132.0 * geo + 50 * 24.0 * coarse_cells
+    coarsest = 4.0 * 256 * coarse_cells if coarsest_op else 50 * 24.0 * coarse_cells
+    per_vc64 = 132.0 * geo + coarsest
     if mixed:
-        per_vc = per_vc64 / 2.0
+        per_vc = 66.0 * geo + (coarsest if coarsest_op else coarsest / 2.0)
         return 168.0 + 48.0 + v_cycles_per_step * per_vc + refinements * 40.0 + 16.0
     return 168.0 + 48.0 + v_cycles_per_step * (per_vc64 + 16.0) + 16.0
 
@@ -578,9 +584,11 @@ def main():
     ap.add_argument("--variants", default="fp64,mixed")
     ap.add_argument("--mixed-inner", type=int, default=2,
                     help="FP32 V-cycles per FP64 refinement in the mixed variant")
-    ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator"], default="operator",
+    ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator", "exact"],
+                    default="exact",
                     help="coarsest level of the mixed variant's FP32 V-cycles: the 50 sweeps, "
-                         "or those sweeps applied as their precomputed 256x256 operator")
+                         "those sweeps applied as their precomputed 256x256 operator, or the "
+                         "level's exact least-squares operator")
     ap.add_argument("--mixed-arith", choices=["float64", "float32"], default="float32",
                     help="arithmetic of the mixed variant's FP32 V-cycle passes: the reference's "
                          "float64 expression, or float32")
@@ -682,7 +690,8 @@ def main():
         r["coarsest_solve"] = coarsest
         r["inner_arithmetic"] = arith if var == "mixed" else None
         r["alg_bytes_per_cell_step"] = algorithmic_bytes_per_cell(
-            r["v_cycles_per_step"], var == "mixed", r["levels"] or 8, r["refinements_per_step"])
+            r["v_cycles_per_step"], var == "mixed", r["levels"] or 8, r["refinements_per_step"],
+            coarsest_op=var == "mixed" and coarsest in ("operator", "exact"))
         r["alg_GBps_whole_step"] = r["alg_bytes_per_cell_step"] * r["cell_steps_per_s"] / 1e9
         results[var] = r
         log(f"variant {var}: {json.dumps(r)}")
@@ -734,7 +743,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plume_{args.N}_mg", "grid": [args.N, args.N],
                    "solver": "multigrid V(2,2) RB-GS, 8 levels, coarsest 50 sweeps, tol 1e-8, "
-                             "rediscretized coarse scale",
+                             "rediscretized coarse scale" + (
+                                 "; mixed: FP32 V-cycles, 16x16 level solved exactly"
+                                 if main_var == "mixed" and
+                                 results[main_var].get("coarsest_solve") == "exact" else ""),
                    "variant": main_var,
                    "coarsest_solve": results[main_var].get("coarsest_solve", "sweeps"),
                    "inner_arithmetic": results[main_var].get("inner_arithmetic")},

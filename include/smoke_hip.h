@@ -249,6 +249,13 @@ typedef struct smk_mg smk_mg;
                                    those sweeps as their precomputed matrix
                                    (the sweeps from a zero guess are linear in
                                    the rhs); other solves use the sweeps     */
+#define SMK_COARSEST_EXACT 2    /* as SMK_COARSEST_OPERATOR, but the matrix is
+                                   the coarsest level's exact least-squares
+                                   solution operator (the limit of the sweeps,
+                                   minus its drift along the level's constant
+                                   null space), built once on the host;
+                                   coarsest_iterations and coarsest_omega do
+                                   not enter (new, mixed precision only)     */
 
 typedef struct smk_mg_params {
   int v_cycles;            /* fixed cycles (tolerance <= 0) or the cap    */

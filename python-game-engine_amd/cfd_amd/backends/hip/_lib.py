@@ -30,6 +30,7 @@ SMK_INNER_F64 = 0
 SMK_INNER_F32 = 1
 SMK_COARSEST_SWEEPS = 0
 SMK_COARSEST_OPERATOR = 1
+SMK_COARSEST_EXACT = 2
 SMK_ADJ_GATHER = 0
 SMK_ADJ_SCATTER = 1
 

@@ -16,8 +16,9 @@ import ctypes
 
 import torch
 
-from ._lib import (SMK_COARSE_REDISCRETIZED, SMK_COARSE_REFERENCE, SMK_COARSEST_OPERATOR,
-                   SMK_COARSEST_SWEEPS, SMK_INNER_F32, SMK_INNER_F64, SMK_PREC_MIXED,
+from ._lib import (SMK_COARSE_REDISCRETIZED, SMK_COARSE_REFERENCE, SMK_COARSEST_EXACT,
+                   SMK_COARSEST_OPERATOR, SMK_COARSEST_SWEEPS, SMK_INNER_F32, SMK_INNER_F64,
+                   SMK_PREC_MIXED,
                    SMK_PREC_NATIVE, SMK_SMOOTH_GS, SMK_SMOOTH_JACOBI, SMK_SMOOTH_RBSOR, MgParams,
                    SolveInfo,
                    call, lib, stream_of)
@@ -131,7 +132,8 @@ _SMOOTHERS = {"jacobi": SMK_SMOOTH_JACOBI, "gauss_seidel": SMK_SMOOTH_GS,
               "rb_sor_gauss_seidel": SMK_SMOOTH_RBSOR}
 _COARSE = {"reference": SMK_COARSE_REFERENCE, "rediscretized": SMK_COARSE_REDISCRETIZED}
 _PREC = {"native": SMK_PREC_NATIVE, "mixed": SMK_PREC_MIXED}
-_COARSEST = {"sweeps": SMK_COARSEST_SWEEPS, "operator": SMK_COARSEST_OPERATOR}
+_COARSEST = {"sweeps": SMK_COARSEST_SWEEPS, "operator": SMK_COARSEST_OPERATOR,
+             "exact": SMK_COARSEST_EXACT}
 _INNER = {"float64": SMK_INNER_F64, "float32": SMK_INNER_F32}
 _GUESS = {"given": 0, "zero": 1}
 

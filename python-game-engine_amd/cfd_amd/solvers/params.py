@@ -71,7 +71,9 @@ class MultigridParams:
     inner_cycles: int = 1  # new
     coarsest_omega: float = 0.0  # new: 0 = smoother_omega; < 0 optimal for the coarsest size
     # new: "sweeps" (the reference's), or "operator": the mixed solver's FP32
-    # V-cycles apply the coarsest sweeps as their precomputed matrix
+    # V-cycles apply the coarsest sweeps as their precomputed matrix, or
+    # "exact": they apply the coarsest level's exact least-squares solution
+    # operator (the limit of the sweeps) instead
     coarsest_solve: str = "sweeps"
     # new: "float64" (the reference's expression, as a float32 field's solve
     # evaluates it) or "float32": the mixed solver's FP32 V-cycles smooth in

@@ -33,6 +33,7 @@
 #include <mutex>
 #include <vector>
 
+#include "smk_coarse_exact.hpp"
 #include "smk_common.hpp"
 
 namespace smk {
@@ -3394,17 +3395,20 @@ static int mg_ensure32(smk_mg* mg) {
   return SMK_OK;
 }
 
-// The float32 operator of the coarsest level's red-black passes (built once
-// per handle and alpha / omega / pass count); null when the coarsest level
-// has more than 256 cells (or the smoother is not red-black).
-inline int ensure_coarse_op(smk_mg* mg, const MgRun& run, double h, const float** out,
-                            hipStream_t s) {
+// The float32 operator of the coarsest level (built once per handle, mask
+// state and alpha / omega / pass count): the red-black passes' matrix, or
+// with `exact` the level's exact least-squares solution operator
+// (coarse_exact_operator, built on the host from the level's flags; the
+// pass count and omega do not enter).  Null when the coarsest level has
+// more than 256 cells (or the smoother is not red-black).
+inline int ensure_coarse_op(smk_mg* mg, const MgRun& run, double h, bool exact,
+                            const float** out, hipStream_t s) {
   *out = nullptr;
   const Level& C = mg->lv[mg->nl - 1];
   if (run.mode != MODE_RB || C.H > 16 || C.W > 16) return SMK_OK;
   const double alpha = calc_alpha(h * (double)(1L << (mg->nl - 1)), run.rho, run.dt);
-  const double omega = level_omega(run, C, true);
-  const int npass = 2 * run.ci;
+  const double omega = exact ? 0.0 : level_omega(run, C, true);
+  const int npass = exact ? -2 : 2 * run.ci;
   if (mg->cop == nullptr) {
     int rc = mg_alloc(reinterpret_cast<void**>(&mg->cop), (size_t)256 * 256 * sizeof(float));
     if (rc) return rc;
@@ -3412,10 +3416,23 @@ inline int ensure_coarse_op(smk_mg* mg, const MgRun& run, double h, const float*
   }
   if (mg->cop_npass != npass || mg->cop_alpha != alpha || mg->cop_omega != omega) {
     const int n = C.H * C.W;
-    hipLaunchKernelGGL(k_coarse_op, dim3(n), dim3(64), 0, s, C.flags, C.H, C.W, alpha, omega,
-                       npass, mg->cop);
-    int rc = check_launch("k_coarse_op");
-    if (rc) return rc;
+    if (exact) {
+      uint8_t fl[256];
+      SMK_CHECK_HIP(hipMemcpyAsync(fl, C.flags, (size_t)n, hipMemcpyDeviceToHost, s));
+      SMK_CHECK_HIP(hipStreamSynchronize(s));
+      std::vector<float> op((size_t)n * n);
+      SMK_REQUIRE(coarse_exact_operator(fl, C.H, C.W, alpha, op.data()) == 0,
+                  "coarsest level %dx%d: no exact operator (inconsistent face flags)", C.H,
+                  C.W);
+      SMK_CHECK_HIP(hipMemcpyAsync(mg->cop, op.data(), op.size() * sizeof(float),
+                                   hipMemcpyHostToDevice, s));
+      SMK_CHECK_HIP(hipStreamSynchronize(s));
+    } else {
+      hipLaunchKernelGGL(k_coarse_op, dim3(n), dim3(64), 0, s, C.flags, C.H, C.W, alpha, omega,
+                         npass, mg->cop);
+      int rc = check_launch("k_coarse_op");
+      if (rc) return rc;
+    }
     mg->cop_alpha = alpha;
     mg->cop_omega = omega;
     mg->cop_npass = npass;
@@ -3434,8 +3451,8 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
   Level& L0 = mg->lv[0];
   MgRun run = make_run(prm, rho, dt);
   mg_prepare_small<float>(mg, 1, run.mode);
-  if (prm->coarsest_solve == SMK_COARSEST_OPERATOR) {
-    rc = ensure_coarse_op(mg, run, h, &run.cop, s);
+  if (prm->coarsest_solve == SMK_COARSEST_OPERATOR || prm->coarsest_solve == SMK_COARSEST_EXACT) {
+    rc = ensure_coarse_op(mg, run, h, prm->coarsest_solve == SMK_COARSEST_EXACT, &run.cop, s);
     if (rc) return rc;
   }
   run.fast = prm->inner_arithmetic == SMK_INNER_F32;
@@ -3926,6 +3943,10 @@ int smk_mg_solve(smk_mg* mg, void* p, const void* div, double rho, double h, dou
   SMK_REQUIRE(prm->coarse_scale == SMK_COARSE_REFERENCE ||
                   prm->coarse_scale == SMK_COARSE_REDISCRETIZED,
               "smk_mg_solve: bad coarse_scale");
+  SMK_REQUIRE(prm->coarsest_solve == SMK_COARSEST_SWEEPS ||
+                  prm->coarsest_solve == SMK_COARSEST_OPERATOR ||
+                  prm->coarsest_solve == SMK_COARSEST_EXACT,
+              "smk_mg_solve: bad coarsest_solve %d", prm->coarsest_solve);
   hipStream_t s = as_stream(stream);
   if (prm->precision == SMK_PREC_MIXED) {
     SMK_REQUIRE(mg->dtype == SMK_F64, "smk_mg_solve: mixed precision needs a float64 handle");

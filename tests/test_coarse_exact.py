@@ -1,0 +1,212 @@
+"""The exact coarsest-level operator (csrc/smk_coarse_exact.hpp, the
+coarsest_solve="exact" option of mixed-precision multigrid), built on the
+CPU and checked against the oracle's own red-black relaxation and residual
+(oracle/smoke_oracle.c orc_rbsor_sweeps / orc_residual, restating
+red_black_sor_gauss_seidel.py:8-27 and residual.py:6-42):
+
+* for a right-hand side with no component along the level's null space, the
+  operator's result is the limit of the oracle's sweeps (up to the constant
+  per pure-Neumann fluid component that the sweeps leave undetermined);
+* for any right-hand side, the oracle's residual of the result is constant
+  on every pure-Neumann component and zero elsewhere (least squares), and the
+  result sums to zero on every such component (least norm): the two
+  conditions that fix the pseudo-inverse solution.
+
+The host code is compiled with g++ into a small test library; the GPU path
+that applies the operator is tested in test_gpu_coarse_operator.py."""
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
+
+import oracle as O
+
+ROOT = Path(__file__).resolve().parents[1]
+CSRC = ROOT / "python-game-engine_amd" / "csrc"
+F_FLUID, F_LEFT, F_RIGHT, F_DOWN, F_UP = 1, 2, 4, 8, 16
+
+SHIM = r"""
+#include "smk_coarse_exact.hpp"
+extern "C" int coarse_exact(const uint8_t* flags, int H, int W, double alpha, float* out) {
+  return smk::coarse_exact_operator(flags, H, W, alpha, out);
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def exact(tmp_path_factory):
+    d = tmp_path_factory.mktemp("coarse_exact")
+    (d / "shim.cpp").write_text(SHIM)
+    so = d / "libce.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", f"-I{CSRC}",
+                    str(d / "shim.cpp"), "-o", str(so)], check=True)
+    lib = ctypes.CDLL(str(so))
+    lib.coarse_exact.restype = ctypes.c_int
+    lib.coarse_exact.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_void_p]
+
+    def run(flags):
+        H, W = flags.shape
+        n = H * W
+        fl = np.ascontiguousarray(flags, dtype=np.uint8)
+        out = np.zeros(n * n, dtype=np.float32)
+        rc = lib.coarse_exact(fl.ctypes.data, H, W, ALPHA, out.ctypes.data)
+        return rc, out.reshape(n, n)  # out[j, i]: cell i for the unit rhs j
+
+    return run
+
+
+ALPHA = O.calculate_alpha(128.0, 1.0, 0.016)  # a 16x16 coarsest level of 2048^2 cells
+
+
+def masks_from_solid(solid):
+    """Face masks of a solid mask: a face is open between two non-solid cells."""
+    H, W = solid.shape
+    fmu = np.zeros((H, W + 1), dtype=bool)
+    fmv = np.zeros((H + 1, W), dtype=bool)
+    fmu[:, 1:-1] = ~(solid[:, :-1] | solid[:, 1:])
+    fmv[1:-1, :] = ~(solid[:-1, :] | solid[1:, :])
+    return fmu, fmv
+
+
+def flags_of(fmu, fmv, is_fluid):
+    """The solvers' flag byte: fluid bit, and a neighbour bit per open face
+    that leads to another cell of the level."""
+    H, W = is_fluid.shape
+    f = np.where(is_fluid, F_FLUID, 0).astype(np.uint8)
+    f[:, 1:] |= np.where(fmu[:, 1:-1], F_LEFT, 0).astype(np.uint8)
+    f[:, :-1] |= np.where(fmu[:, 1:-1], F_RIGHT, 0).astype(np.uint8)
+    f[1:, :] |= np.where(fmv[1:-1, :], F_DOWN, 0).astype(np.uint8)
+    f[:-1, :] |= np.where(fmv[1:-1, :], F_UP, 0).astype(np.uint8)
+    return f
+
+
+def neumann_parts(fmu, fmv, is_fluid):
+    """Indicator vectors of the fluid components (joined by open faces) that
+    have no open face to a non-fluid cell."""
+    H, W = is_fluid.shape
+    idx = np.arange(H * W).reshape(H, W)
+    a = np.concatenate([idx[:, :-1][fmu[:, 1:-1]], idx[:-1, :][fmv[1:-1, :]]])
+    b = np.concatenate([idx[:, 1:][fmu[:, 1:-1]], idx[1:, :][fmv[1:-1, :]]])
+    g = coo_matrix((np.ones(len(a)), (a, b)), shape=(H * W, H * W))
+    _, lab = connected_components(g, directed=False)
+    fl = is_fluid.ravel()
+    parts = []
+    for c in np.unique(lab[fl]):
+        cells = (lab == c) & fl
+        # an open face of a component cell to a cell outside the fluid set
+        leaky = np.any(cells[a] & ~fl[b]) or np.any(cells[b] & ~fl[a])
+        if not leaky:
+            parts.append(cells.astype(float))
+    return parts
+
+
+def cases():
+    H = W = 16
+    ring = np.zeros((H, W), dtype=bool)
+    ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = True
+    out = {"open": ring.copy()}
+    wall = ring.copy()
+    wall[:, 7] = True  # two separate rooms
+    out["two_rooms"] = wall
+    obst = ring.copy()
+    obst[5:9, 4:7] = True
+    obst[10, 10:14] = True
+    obst[12, 2] = True
+    out["obstacles"] = obst
+    rag = np.zeros((10, 12), dtype=bool)
+    rag[0, :] = rag[-1, :] = rag[:, 0] = rag[:, -1] = True
+    rag[4:6, 3:5] = True
+    out["ragged_10x12"] = rag
+    return out
+
+
+def _setup(solid, stale=None):
+    fmu, fmv = masks_from_solid(solid)
+    is_fluid = O.is_fluid_from_masks(fmu, fmv)
+    if stale is not None:  # a cached level's is_fluid that no longer matches the masks
+        is_fluid = is_fluid.copy()
+        for (j, i) in stale:
+            is_fluid[j, i] = False
+    return fmu, fmv, is_fluid
+
+
+ALL = [(k, None) for k in cases()] + [("open", [(5, 5), (9, 3)]), ("two_rooms", [(3, 2)])]
+
+
+@pytest.mark.parametrize("name,stale", ALL)
+def test_limit_of_the_oracle_sweeps(exact, name, stale):
+    solid = cases()[name]
+    fmu, fmv, is_fluid = _setup(solid, stale)
+    H, W = solid.shape
+    rc, op = exact(flags_of(fmu, fmv, is_fluid))
+    assert rc == 0
+    parts = neumann_parts(fmu, fmv, is_fluid)
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(H * W) * is_fluid.ravel()
+    for e in parts:  # remove b's null-space component
+        b -= e * (b @ e) / e.sum()
+    x = (op.astype(np.float64).T @ b).reshape(H, W)
+    p = np.zeros((H, W))
+    omega = O.calculate_optimal_omega(max(H, W))
+    O.rbsor_sweeps(p, b.reshape(H, W), is_fluid, fmu, fmv, ALPHA, 0.25, omega, 4000)
+    p = p.ravel()
+    for e in parts:  # the sweeps' constant per pure-Neumann component is free
+        p -= e * (p @ e) / e.sum()
+    scale = np.abs(p).max()
+    assert scale > 0
+    np.testing.assert_allclose(x.ravel(), p, rtol=0, atol=2e-6 * scale)
+    assert np.all(x.ravel()[~is_fluid.ravel()] == 0.0)
+
+
+@pytest.mark.parametrize("name,stale", ALL)
+def test_least_squares_least_norm(exact, name, stale):
+    solid = cases()[name]
+    fmu, fmv, is_fluid = _setup(solid, stale)
+    H, W = solid.shape
+    rc, op = exact(flags_of(fmu, fmv, is_fluid))
+    assert rc == 0
+    parts = neumann_parts(fmu, fmv, is_fluid)
+    if name in ("open", "obstacles", "ragged_10x12") and stale is None:
+        assert len(parts) == 1
+    if name == "two_rooms" and stale is None:
+        assert len(parts) == 2
+    if stale is not None:  # the stale cells' open faces leak: a nonsingular part
+        assert len(parts) < len(neumann_parts(fmu, fmv, O.is_fluid_from_masks(fmu, fmv)))
+    rng = np.random.default_rng(11)
+    b = (rng.standard_normal(H * W) + 0.3) * is_fluid.ravel()  # with a null-space part
+    x = op.astype(np.float64).T @ b
+    r = np.zeros((H, W))
+    O.compute_residual(x.reshape(H, W), b.reshape(H, W), is_fluid, fmu, fmv, ALPHA, 0.25, r)
+    r = r.ravel()
+    scale = np.abs(ALPHA * b).max()
+    rest = is_fluid.ravel().copy()
+    for e in parts:
+        on = e > 0
+        rest &= ~on
+        # least squares: the residual is the component of alpha*b along e
+        np.testing.assert_allclose(r[on], (ALPHA * b) @ e / e.sum(), rtol=0, atol=1e-5 * scale)
+        # least norm: no component along e
+        assert abs(x @ e) <= 1e-5 * np.abs(x).sum()
+    np.testing.assert_allclose(r[rest], 0.0, atol=1e-5 * scale)
+
+
+def test_isolated_and_empty(exact):
+    fl = np.zeros((4, 4), dtype=np.uint8)
+    rc, op = exact(fl)  # no fluid cell: zero operator
+    assert rc == 0 and not op.any()
+    fl[1, 1] = F_FLUID  # a fluid cell without open faces: its own null space
+    rc, op = exact(fl)
+    assert rc == 0 and not op.any()
+
+
+def test_rejects_one_sided_faces_and_large_levels(exact):
+    fl = np.zeros((4, 4), dtype=np.uint8)
+    fl[1, 1] = F_FLUID | F_RIGHT
+    fl[1, 2] = F_FLUID  # no F_LEFT back
+    assert exact(fl)[0] == -1
+    assert exact(np.zeros((17, 16), dtype=np.uint8))[0] == -1

@@ -128,3 +128,53 @@ def test_float32_inner_arithmetic_differs_from_float64():
     assert np.abs(a - b).max() <= 1e-4 * np.abs(a).max()
     with pytest.raises(ValueError, match="inner_arithmetic must be one of"):
         solve(64, 64, inner_arithmetic="float16", num_levels=3, v_cycles=1)
+
+
+# coarsest_solve = "exact": the FP32 V-cycles apply the coarsest level's exact
+# least-squares operator (its host construction is checked against the
+# oracle's sweeps in test_coarse_exact.py)
+
+def test_exact_coarsest_converges_in_fewer_refinements():
+    """At the same tolerance the exact coarsest level takes fewer refinements
+    than the sweeps' operator, and its result is closer to the converged
+    solution (an FP64 solve to 1e-13) than the FP64 V-cycles' own result at
+    that tolerance: the error left at 1e-8 is the smooth component the 50
+    coarsest sweeps leave behind (measured 1.0e-6 against 2.1e-5 relative)."""
+    kw = dict(num_levels=5, v_cycles=200, inner_cycles=2, block=False)
+    tight, it = solve(256, 256, tolerance=1e-13, **dict(kw, precision="native"))
+    assert it["converged"]
+    res = {}
+    for name, extra in (("native", dict(precision="native")),
+                        ("operator", dict(coarsest_solve="operator")),
+                        ("exact", dict(coarsest_solve="exact"))):
+        got, info = solve(256, 256, tolerance=1e-8, **dict(kw, **extra))
+        assert info["converged"] and info["rel_residual"] <= 1e-8, (name, info)
+        res[name] = (up_to_constant(got, tight), info)
+    assert res["exact"][1]["refinements"] < res["operator"][1]["refinements"], res
+    assert res["exact"][0] <= 0.25 * res["native"][0], res
+
+
+def test_exact_single_level_is_a_direct_solve():
+    """A 16x16 grid of one level: the V-cycle is the operator itself, so each
+    FP64 refinement gains float32's accuracy (~1e-6 or better)."""
+    got, info = solve(16, 16, coarsest_solve="exact", num_levels=1, v_cycles=20,
+                      inner_cycles=1, tolerance=1e-11)
+    assert info["converged"] and info["rel_residual"] <= 1e-11, info
+    assert info["refinements"] <= 4, info
+    ref, iref = solve(16, 16, coarsest_solve="sweeps", num_levels=1, v_cycles=20,
+                      inner_cycles=1, tolerance=1e-11)
+    assert iref["refinements"] > info["refinements"], (iref, info)
+
+
+def test_exact_operator_cache_follows_the_mode():
+    """One handle alternating operator kinds gives each kind's fresh-handle
+    result bit for bit."""
+    from cfd_amd.backends.hip import solvers
+
+    mg = solvers.MultigridSolver()
+    kw = dict(num_levels=5, v_cycles=3, inner_cycles=1)
+    fresh = {cs: solve(256, 256, coarsest_solve=cs, **kw)[0] for cs in ("exact", "operator")}
+    for cs in ("exact", "operator", "exact"):
+        got, _ = solve(256, 256, mg, coarsest_solve=cs, **kw)
+        np.testing.assert_array_equal(got, fresh[cs])
+    assert not np.array_equal(fresh["exact"], fresh["operator"])

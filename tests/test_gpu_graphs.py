@@ -32,7 +32,7 @@ def _same(a, b, where):
 
 
 @pytest.mark.parametrize("overrides", [None, {"precision": "mixed", "inner_cycles": 2,
-                                              "coarsest_solve": "operator",
+                                              "coarsest_solve": "exact",
                                               "inner_arithmetic": "float32"}])
 def test_graph_stages_match_eager_steps(overrides):
     sc = scenes.plume_2048_mg(N=256, steps=8)

@@ -138,8 +138,8 @@ def test_mixed_rbsor_matches_fp64_to_tolerance(N):
 
 
 # the bench line's headline variant: two FP32 V-cycles per FP64 refinement,
-# the coarsest level applied as its operator, float32 pass arithmetic
-HEADLINE_MIXED = {"precision": "mixed", "inner_cycles": 2, "coarsest_solve": "operator",
+# the coarsest level's exact operator, float32 pass arithmetic
+HEADLINE_MIXED = {"precision": "mixed", "inner_cycles": 2, "coarsest_solve": "exact",
                   "inner_arithmetic": "float32"}
 
 
@@ -148,10 +148,10 @@ def test_headline_mixed_variant_against_fp64_oracle(tol, bound):
     """The headline configuration of the 2048x2048 workload at 128x128, 5
     steps, against the all-FP64 oracle rollout solved to the same tolerance.
     Every solve reaches the tolerance in the FP64 residual.  At the
-    workload's 1e-8 the fields differ from the oracle's by ~1e-7 relative L2
-    (measured 1.6e-7 density, 8e-8 velocity): two solutions each meeting
-    1e-8, not FP32 rounding, since at 1e-11 the difference falls with the
-    tolerance (bound 1e-9)."""
+    workload's 1e-8 the fields differ from the oracle's by a few 1e-7
+    relative L2 (measured 3.8e-7 density, 1.5e-7 velocity): two solutions
+    each meeting 1e-8, not FP32 rounding, since at 1e-11 the difference falls
+    with the tolerance (bound 1e-9)."""
     sc = scenes.plume_2048_mg(N=128, steps=5)
     sim = scenes.make_simulation(sc, dtype="float64",
                                  solver_overrides=dict(HEADLINE_MIXED, tolerance=tol))
