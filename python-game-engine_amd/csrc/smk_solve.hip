@@ -2652,38 +2652,62 @@ inline double optimal_omega(int size) {
 // catches up.  While polling, the stream is queried now and then, so an
 // error on the stream ends the wait; a loop that runs out of iterations
 // reads the state after a stream synchronisation.
+//
+// The no-op iteration is not free (each of its launches still costs ~2 us of
+// dispatch), so with tol > 0 the loop predicts: when the last two outcomes
+// read, r2 then r1, extrapolate geometrically to r1 * (r1 / r2) <= tol for
+// the iteration in flight, it reads that iteration's outcome before queueing
+// the next one.  A right guess saves the no-op iteration for the host's
+// return latency; a wrong one costs the GPU that latency once.  The
+// iterations and their results are the same either way.
 template <typename F>
 int run_gated_ring(SolveState* d_state, RingSlot* h_ring, RingSlot* d_ring, unsigned* seq,
-                   int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
+                   int max_iter, double tol, F&& iter, SolveState* fin, hipStream_t s) {
   volatile RingSlot* ring = h_ring;
   const unsigned base = *seq;
   *seq = base + (unsigned)max_iter + 2u;
   // while profiling, wait for each iteration before queueing the next, so
   // that no gated-off launch enters the per-kernel averages
   const int lag = g_prof_on ? 0 : 1;
-  for (int k = 0; k <= max_iter; ++k) {
-    const int slot = k & 3;
-    int rc = iter(k, d_ring + slot, base + (unsigned)k);
-    if (rc) return rc;
-    if (k < lag) continue;
-    const int prev = (k - lag) & 3;
-    const unsigned want = base + (unsigned)(k - lag);
-    for (unsigned spin = 1; ring[prev].seq != want; ++spin) {
+  int known = -1;               // the last iteration whose outcome was read
+  double r1 = -1.0, r2 = -1.0;  // the last two relative residuals read
+  // read iteration it's outcome; *done = 1 when it reports the solve done
+  auto read_outcome = [&](int it, int* done) -> int {
+    const int slot = it & 3;
+    const unsigned want = base + (unsigned)it;
+    for (unsigned spin = 1; ring[slot].seq != want; ++spin) {
       if ((spin & 1023u) == 0u) {
         const hipError_t e = hipStreamQuery(s);
-        if (e == hipSuccess && ring[prev].seq != want)
+        if (e == hipSuccess && ring[slot].seq != want)
           return fail((int)hipErrorUnknown, "gated solve: check %u never published", want);
         if (e != hipSuccess && e != hipErrorNotReady)
           return fail((int)e, "gated solve: %s", hipGetErrorString(e));
       }
     }
     __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    if (ring[prev].active == 0) {
-      fin->active = ring[prev].active;
-      fin->cycles = ring[prev].cycles;
-      fin->rel = ring[prev].rel;
-      return SMK_OK;
+    known = it;
+    r2 = r1;
+    r1 = ring[slot].rel;
+    *done = ring[slot].active == 0;
+    if (*done) {
+      fin->active = ring[slot].active;
+      fin->cycles = ring[slot].cycles;
+      fin->rel = ring[slot].rel;
     }
+    return SMK_OK;
+  };
+  for (int k = 0; k <= max_iter; ++k) {
+    int done = 0, rc;
+    if (lag > 0 && k - 1 > known && tol > 0.0 && r1 > 0.0 && r2 > 0.0 &&
+        r1 * (r1 / r2) <= tol) {
+      rc = read_outcome(k - 1, &done);
+      if (rc || done) return rc;
+    }
+    rc = iter(k, d_ring + (k & 3), base + (unsigned)k);
+    if (rc) return rc;
+    if (k - lag < 0 || k - lag <= known) continue;
+    rc = read_outcome(k - lag, &done);
+    if (rc || done) return rc;
   }
   SMK_CHECK_HIP(hipStreamSynchronize(s));
   SMK_CHECK_HIP(hipMemcpy(fin, d_state, sizeof(SolveState), hipMemcpyDeviceToHost));
@@ -2723,7 +2747,7 @@ int sweeps_gated(SolverWs& w, int iterations, int check_every, double tol, F&& c
   rc = check_launch("k_state_init");
   if (rc) return rc;
   SolveState fin{};
-  rc = run_gated_ring(w.state, g->h_ring, g->d_ring, &g->seq, nchunks + 1,
+  rc = run_gated_ring(w.state, g->h_ring, g->d_ring, &g->seq, nchunks + 1, tol,
                       [&](int k, RingSlot* slot, unsigned sq) -> int {
     int r = SMK_OK;
     if (k > 0) {
@@ -3261,9 +3285,9 @@ int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, do
 // k is already queued, so the GPU never waits for the host; the one
 // iteration queued after the last useful one is gated off and does nothing.
 template <typename F>
-int run_gated(smk_mg* mg, int max_iter, F&& iter, SolveState* fin, hipStream_t s) {
-  return run_gated_ring(mg->d_state, mg->h_ring, mg->d_ring, &mg->ring_seq, max_iter, iter, fin,
-                        s);
+int run_gated(smk_mg* mg, int max_iter, double tol, F&& iter, SolveState* fin, hipStream_t s) {
+  return run_gated_ring(mg->d_state, mg->h_ring, mg->d_ring, &mg->ring_seq, max_iter, tol, iter,
+                        fin, s);
 }
 
 // Gated iterations need the finest level's ping-pong flips per V-cycle to be
@@ -3314,7 +3338,8 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
     rc = check_launch("k_state_init");
     if (rc) return rc;
     SolveState fin{};
-    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, RingSlot* slot, unsigned sq) -> int {
+    rc = run_gated(mg, prm->v_cycles + 1, prm->tolerance,
+                   [&](int k, RingSlot* slot, unsigned sq) -> int {
       int nparts = -1;
       int r = SMK_OK;
       if (k > 0) r = mg_vcycle<T>(mg, 0, false, grun, h, zg && k == 1, mg->partial, &nparts, s);
@@ -3493,7 +3518,8 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     // nothing, so the result is in buf[executed refinements & 1]
     double* buf[2] = {p, reinterpret_cast<double*>(L0.p[1])};
     SolveState fin{};
-    rc = run_gated(mg, prm->v_cycles + 1, [&](int k, RingSlot* slot, unsigned sq) -> int {
+    rc = run_gated(mg, prm->v_cycles + 1, prm->tolerance,
+                   [&](int k, RingSlot* slot, unsigned sq) -> int {
       int r = SMK_OK;
       float* defect = reinterpret_cast<float*>(L0.rhs32);
       if (k > 0) {
