@@ -1646,50 +1646,59 @@ __device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t*
 // x = op b for a coarsest level of n <= 256 cells: op[j * n + i] is the
 // result in cell i of the coarsest red-black passes from a zero guess with
 // rhs = unit vector j (k_coarse_op).  The passes are linear in the rhs, so
-// this is their result for b up to rounding.  Thread t covers rows
-// 4 (t % 64) .. + 3 of the columns j = t / 64 (mod NT / 64): every load of
-// the matrix is a float4 issued before any is used (one memory latency for
-// the whole product, where a column-by-column loop waits once per column);
-// the NT / 64 partial sums meet in LDS.
+// this is their result for b up to rounding (k_coarse_op; or the exact
+// solution operator, coarse_exact_operator).  Thread t covers rows
+// 4 (t % 64) .. + 3 of a column set: every load of a set's matrix columns is
+// a float4 issued before any is used (one memory latency per set, where a
+// column-by-column loop waits once per column); the sets' partial sums meet
+// in LDS.
 template <typename T, int NT>
 __device__ __forceinline__ void coarsest_apply_op(const float* __restrict__ op, const T* b, T* x,
                                                   int n) {
-  constexpr int G = NT / 64;            // column groups
-  constexpr int JMAX = (256 + G - 1) / G;  // columns per group at most
+  // 16 column sets j = g (mod 16), one float partial per (set, row) summed
+  // over the set's columns in order, then the 16 partials of a row in double
+  // in set order: the order of 1024 threads, one set per wavefront.  With
+  // fewer threads a wavefront takes several sets in turn, so the result does
+  // not depend on NT.
+  constexpr int G = 16;
+  constexpr int JMAX = 256 / G;  // columns per set at most
+  static_assert(NT % 64 == 0 && NT <= 64 * G, "whole wavefronts, at most one set each");
   __shared__ float part[G][256];
-  const int q = threadIdx.x % 64, g = threadIdx.x / 64;
+  const int q = threadIdx.x % 64;
   const int i0 = 4 * q;
-  if ((n & 3) == 0) {
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 m[JMAX];
-    float bj[JMAX];
+  for (int g = threadIdx.x / 64; g < G; g += NT / 64) {
+    if ((n & 3) == 0) {
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 m[JMAX];
+      float bj[JMAX];
 #pragma unroll
-    for (int k = 0; k < JMAX; ++k) {
-      const int j = g + k * G;
-      const bool ok = j < n && i0 < n;
-      m[k] = ok ? *reinterpret_cast<const float4*>(op + (size_t)j * n + i0)
-                : make_float4(0.f, 0.f, 0.f, 0.f);
-      bj[k] = j < n ? (float)b[j] : 0.f;
-    }
+      for (int k = 0; k < JMAX; ++k) {
+        const int j = g + k * G;
+        const bool ok = j < n && i0 < n;
+        m[k] = ok ? *reinterpret_cast<const float4*>(op + (size_t)j * n + i0)
+                  : make_float4(0.f, 0.f, 0.f, 0.f);
+        bj[k] = j < n ? (float)b[j] : 0.f;
+      }
 #pragma unroll
-    for (int k = 0; k < JMAX; ++k) {
-      acc.x += m[k].x * bj[k];
-      acc.y += m[k].y * bj[k];
-      acc.z += m[k].z * bj[k];
-      acc.w += m[k].w * bj[k];
-    }
-    part[g][i0] = acc.x;
-    part[g][i0 + 1] = acc.y;
-    part[g][i0 + 2] = acc.z;
-    part[g][i0 + 3] = acc.w;
-  } else {  // rows one at a time
+      for (int k = 0; k < JMAX; ++k) {
+        acc.x += m[k].x * bj[k];
+        acc.y += m[k].y * bj[k];
+        acc.z += m[k].z * bj[k];
+        acc.w += m[k].w * bj[k];
+      }
+      part[g][i0] = acc.x;
+      part[g][i0 + 1] = acc.y;
+      part[g][i0 + 2] = acc.z;
+      part[g][i0 + 3] = acc.w;
+    } else {  // rows one at a time
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = i0 + r;
-      float acc = 0.f;
-      for (int j = g; j < n; j += G)
-        if (i < n) acc += op[(size_t)j * n + i] * (float)b[j];
-      part[g][i] = acc;
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + r;
+        float acc = 0.f;
+        for (int j = g; j < n; j += G)
+          if (i < n) acc += op[(size_t)j * n + i] * (float)b[j];
+        part[g][i] = acc;
+      }
     }
   }
   __syncthreads();

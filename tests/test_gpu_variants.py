@@ -179,3 +179,19 @@ def test_tile_variant_mixed_float32_arithmetic(which, name):
         got, info = run_mg("float64", **kw)
     np.testing.assert_array_equal(got, want)
     assert info["refinements"] == winfo["refinements"]
+
+
+@pytest.mark.parametrize("coarsest", ["operator", "exact"])
+@pytest.mark.parametrize("name", SMALL_VARIANTS)
+def test_small_variant_mixed_coarsest_matrix(coarsest, name):
+    """The 16x16 coarsest level applied as a matrix (the sweeps' operator or
+    the exact one) inside the FP32 V-cycles: every small-level workgroup size
+    gives the default's pressure, bit for bit (the product sums in an order
+    that does not depend on the workgroup size)."""
+    kw = dict(num_levels=5, v_cycles=6, precision="mixed", inner_cycles=2,
+              inner_arithmetic="float32", coarsest_solve=coarsest)
+    want, winfo = run_mg("float64", 256, 256, **kw)
+    with variant("float32", "small", name):
+        got, info = run_mg("float64", 256, 256, **kw)
+    np.testing.assert_array_equal(got, want)
+    assert info["refinements"] == winfo["refinements"]
