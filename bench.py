@@ -581,6 +581,9 @@ def main():
     ap.add_argument("--N", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default tile variants (profiled runs: a profiler's overhead "
+                         "distorts the autotuner's timings)")
     ap.add_argument("--variants", default="fp64,mixed")
     ap.add_argument("--mixed-inner", type=int, default=2,
                     help="FP32 V-cycles per FP64 refinement in the mixed variant")
@@ -685,7 +688,8 @@ def main():
         coarsest = args.mixed_coarsest if var == "mixed" else "sweeps"
         arith = args.mixed_arith if var == "mixed" else "float64"
         r, sims[var] = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
-                                   args.warmup, inner=inner, coarsest=coarsest, arith=arith)
+                                   args.warmup, inner=inner, coarsest=coarsest, arith=arith,
+                                   tune=not args.no_tune)
         r["inner_cycles_per_refinement"] = inner
         r["coarsest_solve"] = coarsest
         r["inner_arithmetic"] = arith if var == "mixed" else None
