@@ -337,313 +337,58 @@ __global__ __launch_bounds__(NT) void k_smooth_fine(const SmoothArgs<T> a) {
 // per-cell arithmetic is the same expression in the same order.  Residuals
 // for the restriction are computed by the owning lane and passed through the
 // p array once the passes are done.
-__device__ __forceinline__ bool fbit(const uint32_t* fw, int k, uint32_t bit) {
-  return (fw[k >> 2] & (bit << (8 * (k & 3)))) != 0u;
+//
+// These launches are bound by vector-ALU issue as much as by memory (a
+// wave64 VALU instruction occupies its SIMD for four cycles), so whatever
+// depends only on a row is computed once per wavefront: the wave index is
+// made uniform, row indices, clamps and row base addresses then live in
+// scalar registers, and a lane's loads and stores add only its fixed column
+// offset to a scalar row base.
+// The index of the calling thread's wavefront, uniform over it.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
-// Per-tile state of a lane: the rhs and flags of its 2K cells by colour, and
-// the wave-uniform mask of pairs whose cells are fluid with all four faces
-// open in every lane (the stencil then needs no selects: a scalar branch).
+// Element `off` bytes into a row whose base is uniform over the wavefront.
+template <typename T>
+__device__ __forceinline__ const T& row_at(const T* row, unsigned off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(row) + off);
+}
+template <typename T>
+__device__ __forceinline__ T& row_at(T* row, unsigned off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(row) + off);
+}
+
+__device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+// Per-tile state of a lane: the rhs of its 2K cells by colour, and the
+// wave-uniform mask of pairs whose cells are fluid with all four faces open
+// in every lane (the stencil then needs no selects: a scalar branch).  The
+// flags of the other pairs are read from the LDS copy when needed (cells at
+// walls and obstacles), so no lane masks stay live across the passes.
 template <typename T, int RH, int NT>
 struct RTile {
   static constexpr int RW = 64;
   static constexpr int NW = NT / 64;
   static_assert(RH % (2 * NW) == 0, "row pairs must split evenly over the waves");
   static constexpr int K = RH / (2 * NW);  // row pairs per lane
-  static constexpr int KW = (K + 3) / 4;   // packed flag words per colour
-  T rb[2][K];          // rhs of the colour-c cell of pair k
-  uint32_t fw[2][KW];  // flags of the colour-c cell of pair k, byte k & 3 of word k >> 2
+  T rb[2][K];  // rhs of the colour-c cell of pair k
   uint32_t open[2];
 };
 
-// Raw loads of the tile's rows for the lane's pairs h0 .. h0+KH-1 (clamped
-// into the grid, unconditional: a lane issues them back to back).
-template <typename T, int RH, int NT, int KH>
-__device__ __forceinline__ void rtile_issue(const SmoothArgs<T>& a, int bx, int by, int h0,
-                                            T (&pv)[KH][2], T (&rv)[KH][2],
-                                            uint32_t (&fv)[KH][2]) {
-  using RT = RTile<T, RH, NT>;
-  const int R = a.R;
-  const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
-  const int gxc = gx < 0 ? 0 : (gx >= a.W ? a.W - 1 : gx);
-#pragma unroll
-  for (int k = 0; k < KH; ++k)
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
-      const int gyc = gy < 0 ? 0 : (gy >= a.H ? a.H - 1 : gy);
-      const size_t g = (size_t)gyc * a.W + gxc;
-      fv[k][q] = a.flags[g];
-      rv[k][q] = a.rhs[g];
-      pv[k][q] = a.zero_init ? (T)0 : a.p_in[g];
-    }
-}
-
-// Apply the prolonged correction (spatch: the staged coarse patch, or null),
-// zero what lies outside the grid, and place p and the flags in LDS and the
-// rhs and flags in the lane's registers.
-template <typename T, int RH, int NT>
-__device__ __forceinline__ T prolong_at(const SmoothArgs<T>& a, const T* spatch, int bx, int by,
-                                        int gx, int gy);
-
-template <typename T, int RH, int NT, int KH>
-__device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int by, int h0,
-                                             T (&pv)[KH][2], const T (&rv)[KH][2],
-                                             const uint32_t (&fv)[KH][2], T* sp, uint8_t* sf,
-                                             RTile<T, RH, NT>& t, const T* spatch) {
-  using RT = RTile<T, RH, NT>;
-  constexpr uint32_t F_ALL = F_FLUID | F_LFLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
-  const int R = a.R;
-  const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int px = lane & 1;
-  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
-  const bool colin = gx >= 0 && gx < a.W;
-  if (spatch != nullptr) {  // the prolonged correction, on fluid cells
-#pragma unroll
-    for (int k = 0; k < KH; ++k)
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
-        const T corr = prolong_at<T, RH, NT>(a, spatch, bx, by, gx, gy);
-        pv[k][q] = (fv[k][q] & F_LFLUID) ? pv[k][q] + corr : pv[k][q];
-      }
-  }
-#pragma unroll
-  for (int k = 0; k < KH; ++k) {
-    uint32_t f[2];
-    T r[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int ly = 2 * (wave + (h0 + k) * RT::NW) + q;
-      const int gy = gy0 + ly;
-      const bool in = colin && gy >= 0 && gy < a.H;
-      const int o = ly * RT::RW + lane;
-      f[q] = in ? fv[k][q] : 0u;
-      r[q] = in ? rv[k][q] : (T)0;
-      sp[o] = in ? pv[k][q] : (T)0;
-      sf[o] = (uint8_t)f[q];
-    }
-    // row 2m + q has colour (q + x) & 1
-    const int kk = h0 + k;
-    t.rb[0][kk] = px ? r[1] : r[0];
-    t.rb[1][kk] = px ? r[0] : r[1];
-    const uint32_t f0 = px ? f[1] : f[0], f1 = px ? f[0] : f[1];
-    t.fw[0][kk >> 2] |= f0 << (8 * (kk & 3));
-    t.fw[1][kk >> 2] |= f1 << (8 * (kk & 3));
-    if (__ballot((f0 & F_ALL) == F_ALL) == ~0ull) t.open[0] |= 1u << kk;
-    if (__ballot((f1 & F_ALL) == F_ALL) == ~0ull) t.open[1] |= 1u << kk;
-  }
-}
-
-template <typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_reset(RTile<T, RH, NT>& t) {
-  t.open[0] = t.open[1] = 0u;
-#pragma unroll
-  for (int c = 0; c < 2; ++c)
-#pragma unroll
-    for (int q = 0; q < RTile<T, RH, NT>::KW; ++q) t.fw[c][q] = 0u;
-}
-
-// OPEN: every pair of the lane's colour-C cells is open in every lane (no
-// selects), decided once per pass rather than per cell, so the pass is one
-// straight run of code.  The LDS reads of a batch of KB cells are issued
-// before any is used; KB = 1 measured fastest (larger batches raise the
-// registers past the occupancy of three 512-thread workgroups per CU).  A
-// batch's new values are stored before the next batch reads: a cell's reads
-// are its own value and cells of the other colour, which the pass does not
-// write.
-template <int C, bool OPEN, bool FM, typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                              T* sp, int lo) {
-  using RT = RTile<T, RH, NT>;
-  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
-  constexpr int KB = 1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int par = (lane + C) & 1;
-  const bool xin = lane >= lo && lane < RW - lo;
-#pragma unroll
-  for (int k0 = 0; k0 < K; k0 += KB) {
-    T ce[KB], lv[KB], rv[KB], dv[KB], uv[KB];
-    int oo[KB];
-    bool valid[KB];
-#pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      const int ly = 2 * (wave + (k0 + j) * NW) + par;
-      valid[j] = xin && ly >= lo && ly < RH - lo;
-      const int o = valid[j] ? ly * RW + lane : RW + 1;
-      oo[j] = o;
-      ce[j] = sp[o];
-      lv[j] = sp[o - 1];
-      rv[j] = sp[o + 1];
-      dv[j] = sp[o - RW];
-      uv[j] = sp[o + RW];
-    }
-#pragma unroll
-    for (int j = 0; j < KB; ++j) {
-      const int k = k0 + j;
-      T ns;
-      bool fl;
-      if (OPEN) {
-        ns = lv[j] + rv[j] + dv[j] + uv[j];
-        fl = true;
-      } else {
-        const uint32_t* fwc = t.fw[C];
-        const T l = fbit(fwc, k, F_LEFT) ? lv[j] : ce[j];
-        const T r = fbit(fwc, k, F_RIGHT) ? rv[j] : ce[j];
-        const T d = fbit(fwc, k, F_DOWN) ? dv[j] : ce[j];
-        const T u = fbit(fwc, k, F_UP) ? uv[j] : ce[j];
-        ns = l + r + d + u;
-        fl = fbit(fwc, k, F_FLUID);
-      }
-      T nv;
-      if constexpr (FM && sizeof(T) == 4)
-        nv = relax_f(ce[j], ns, t.rb[C][k], (float)a.alpha, (float)a.omega);
-      else
-        nv = (T)relax_value(0, ce[j], ns, t.rb[C][k], a.alpha, a.omega);
-      if (valid[j] && fl) sp[oo[j]] = nv;
-    }
-  }
-}
-
-template <int C, bool FM, typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_sweep(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                            T* sp, int lo) {
-  constexpr uint32_t ALL = (1u << RTile<T, RH, NT>::K) - 1u;
-  if ((t.open[C] & ALL) == ALL)  // wave-uniform
-    rtile_sweep_k<C, true, FM>(a, t, sp, lo);
-  else
-    rtile_sweep_k<C, false, FM>(a, t, sp, lo);
-}
-
-template <bool FM, typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_passes_m(const SmoothArgs<T>& a,
-                                               const RTile<T, RH, NT>& t, T* sp) {
-  for (int pass = 0; pass < a.npass; ++pass) {
-    if (((a.first_colour + pass) & 1) == 0)
-      rtile_sweep<0, FM>(a, t, sp, pass + 1);
-    else
-      rtile_sweep<1, FM>(a, t, sp, pass + 1);
-    __syncthreads();
-  }
-}
-
-// npass colour passes; the valid window shrinks by one cell per pass.
-template <typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_passes(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                             T* sp) {
-  if (sizeof(T) == 4 && a.fast_math)
-    rtile_passes_m<true>(a, t, sp);
-  else
-    rtile_passes_m<false>(a, t, sp);
-}
-
-// Write the tile interior of p; then the residuals of the lane's own cells
-// in the output window feed the restriction (through the p array) and the
-// residual norm.
-template <typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                               int bx, int by, int part, T* sp,
-                                               const uint8_t* sf, double* red) {
-  using RT = RTile<T, RH, NT>;
-  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
-  const int R = a.R;
-  const int TX = RW - 2 * R, TY = RH - 2 * R;
-  const int H = a.H, W = a.W;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
-  const bool colin = gx >= 0 && gx < W;
-  const bool xout = lane >= R && lane < R + TX && colin;
-  for (int ly = R + wave; ly < R + TY; ly += NW) {
-    const int gy = gy0 + ly;
-    if (xout && gy < H) a.p_out[(size_t)gy * W + gx] = sp[ly * RW + lane];
-  }
-  if (a.coarse_rhs == nullptr && a.norm_partial == nullptr) return;
-
-  T res[2][K];
-  double acc = 0.0;
-  constexpr uint32_t ALL = (1u << K) - 1u;
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const int par = (lane + c) & 1;
-    // as in rtile_sweep_k: one wave-uniform decision for the colour's pairs
-    const bool open_all = (t.open[c] & ALL) == ALL;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int ly = 2 * (wave + k * NW) + par;
-      const bool win = xout && ly >= R && ly < R + TY && gy0 + ly < H;
-      const int o = win ? ly * RW + lane : RW + 1;
-      const T ce = sp[o];
-      const T lv = sp[o - 1], rv = sp[o + 1], dv = sp[o - RW], uv = sp[o + RW];
-      T ns;
-      bool fl;
-      if (open_all) {
-        ns = lv + rv + dv + uv;
-        fl = true;
-      } else {
-        const uint32_t* fwc = t.fw[c];
-        const T l = fbit(fwc, k, F_LEFT) ? lv : ce;
-        const T r = fbit(fwc, k, F_RIGHT) ? rv : ce;
-        const T d = fbit(fwc, k, F_DOWN) ? dv : ce;
-        const T u = fbit(fwc, k, F_UP) ? uv : ce;
-        ns = l + r + d + u;
-        fl = fbit(fwc, k, F_LFLUID);
-      }
-      const double rr = (sizeof(T) == 4 && a.fast_math)
-                            ? (double)residual_f((float)ce, (float)ns, (float)t.rb[c][k],
-                                                 (float)a.alpha)
-                            : residual_value(ce, ns, t.rb[c][k], a.alpha);
-      res[c][k] = (T)rr;
-      acc += (win && fl) ? rr * rr : 0.0;
-    }
-  }
-
-  if (a.coarse_rhs != nullptr) {
-    __syncthreads();  // every lane has read p
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int par = (lane + c) & 1;
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int ly = 2 * (wave + k * NW) + par;
-        if (xout && ly >= R && ly < R + TY && gy0 + ly < H) sp[ly * RW + lane] = res[c][k];
-      }
-    }
-    __syncthreads();
-    const int CX = TX / 2, CY = TY / 2;
-    const int Wc = W / 2, Hc = H / 2;
-    const int cx = lane & 31;
-    for (int cyb = 2 * wave; cyb < CY; cyb += 2 * NW) {
-      const int cy = cyb + (lane >> 5);
-      if (cy >= CY || cx >= CX) continue;
-      const int Jc = by * CY + cy, Ic = bx * CX + cx;
-      if (Jc >= Hc || Ic >= Wc) continue;
-      // the children's residuals summed in float64 over the fluid children
-      // in the (dj, di) order of smooth_tile64_body
-      double total = 0.0;
-      int count = 0;
-#pragma unroll
-      for (int dj = 0; dj < 2; ++dj)
-#pragma unroll
-        for (int di = 0; di < 2; ++di) {
-          const int o = (R + 2 * cy + dj) * RW + R + 2 * cx + di;
-          const bool fl = sf[o] & F_LFLUID;
-          const T rv = sp[o];
-          total += fl ? (double)rv : 0.0;
-          count += fl ? 1 : 0;
-        }
-      const T mean = count > 0 ? (T)(total / (double)count) : (T)0;
-      a.coarse_rhs[(size_t)Jc * Wc + Ic] = (T)((double)mean / a.coarse_div);
-    }
-  }
-
-  if (a.norm_partial != nullptr) {
-    const double tot = block_sum<NT>(acc, red);
-    if (threadIdx.x == 0) a.norm_partial[part] = tot;
-  }
-}
+// p is staged in an LDS array of RH + 2 rows: the region's rows 0 .. RH-1
+// and one row of padding above and below, so that every cell of the region
+// can read its four neighbours in range.  The passes then update every cell
+// of the region, not only those of the shrinking exact window: after pass k
+// the cells at distance >= k from the region's edge are exact whatever the
+// others hold (an exact cell reads only cells at distance >= k - 1 of the
+// other colour), so the extra updates change nothing that is written out,
+// and no lane or pass needs a window test.
+template <typename T, int RH>
+struct SPad {
+  static constexpr int N = (RH + 2) * 64;  // elements of the LDS array
+  static constexpr int OFF = 64;           // element of region row 0, column 0
+};
 
 // The coarse-grid patch the bilinear prolongation of a 64 x RH region reads:
 // region column gx samples the coarse grid at x = (gx + 0.5) / 2 - 0.5, so the
@@ -668,9 +413,7 @@ __device__ __forceinline__ void cpatch_issue(const SmoothArgs<T>& a, int bx, int
   for (int i = 0; i < CP::PER; ++i) {
     const int idx = threadIdx.x + i * NT;
     const int py = idx / CP::PW, px = idx - py * CP::PW;
-    int gy = cy0 + py, gx = cx0 + px;
-    gy = gy < 0 ? 0 : (gy >= a.Hc ? a.Hc - 1 : gy);
-    gx = gx < 0 ? 0 : (gx >= a.Wc ? a.Wc - 1 : gx);
+    const int gy = clampi(cy0 + py, a.Hc), gx = clampi(cx0 + px, a.Wc);
     v[i] = idx < CP::N ? a.coarse_e[(size_t)gy * a.Wc + gx] : (T)0;
   }
 }
@@ -685,54 +428,360 @@ __device__ __forceinline__ void cpatch_store(const T (&v)[CPatch<T, RH, NT>::PER
   }
 }
 
-// The bilinear prolongation of the staged coarse patch at fine cell (gx, gy).
-// The sample position of fine index g is g / 2 - 0.25, never an integer, so
+// The bilinear prolongation of the staged coarse patch, sample_field's value:
+// the sample position of fine index g is g / 2 - 0.25, never an integer, so
 // sample_field's floor and ceil are (g - 1) >> 1 and one more; with the same
 // clamping, weights (x minus the clamped floor, exact) and lerps (the first
-// two differences in T) the value is sample_field's.
+// two differences in T) the value is sample_field's.  The x part depends only
+// on the lane's column (PCol, once per tile); the lerp along x of one coarse
+// row (prolong_xlerp) is shared by the fine rows that sample that row: fine
+// rows g and g + 1 of a row pair read coarse rows r, r + 1 and, for even g,
+// r + 2 (r = (g - 1) >> 1), three x-lerps for two cells instead of four.
+struct PCol {
+  int c0, c1;  // patch columns of the lane's clamped coarse floor and ceil
+  double tx;
+};
+
 template <typename T, int RH, int NT>
-__device__ __forceinline__ T prolong_at(const SmoothArgs<T>& a, const T* spatch, int bx, int by,
-                                        int gx, int gy) {
-  constexpr int PW = CPatch<T, RH, NT>::PW;
-  const int R = a.R, TX = 64 - 2 * R, TY = RH - 2 * R;
-  const int cx0 = (bx * TX - R - 1) >> 1, cy0 = (by * TY - R - 1) >> 1;
-  const int xf = (gx - 1) >> 1, yf = (gy - 1) >> 1;
-  const int x0 = min(max(xf, 0), a.Wc - 1), x1 = min(max(xf + 1, 0), a.Wc - 1);
-  const int y0 = min(max(yf, 0), a.Hc - 1), y1 = min(max(yf + 1, 0), a.Hc - 1);
-  const double tx = ((gx + 0.5) * 0.5 - 0.5) - (double)x0;
-  const double ty = ((gy + 0.5) * 0.5 - 0.5) - (double)y0;
-  const int r0 = (y0 - cy0) * PW, r1 = (y1 - cy0) * PW;
-  const T c00 = spatch[r0 + x0 - cx0], c10 = spatch[r0 + x1 - cx0];
-  const T c01 = spatch[r1 + x0 - cx0], c11 = spatch[r1 + x1 - cx0];
-  const double c0 = (double)c00 + (double)(T)(c10 - c00) * tx;
-  const double c1 = (double)c01 + (double)(T)(c11 - c01) * tx;
-  return (T)(c0 + (c1 - c0) * ty);
+__device__ __forceinline__ PCol prolong_col(const SmoothArgs<T>& a, int bx) {
+  const int R = a.R, TX = 64 - 2 * R;
+  const int cx0 = (bx * TX - R - 1) >> 1;
+  const int gx = bx * TX - R + (int)(threadIdx.x & 63);
+  const int xf = (gx - 1) >> 1;
+  const int x0 = clampi(xf, a.Wc), x1 = clampi(xf + 1, a.Wc);
+  PCol c;
+  c.c0 = x0 - cx0;
+  c.c1 = x1 - cx0;
+  c.tx = ((gx + 0.5) * 0.5 - 0.5) - (double)x0;
+  return c;
+}
+
+template <typename T, int RH, int NT>
+__device__ __forceinline__ double prolong_xlerp(const SmoothArgs<T>& a, const T* spatch,
+                                                const PCol& pc, int cy0, int y) {
+  const int r = (clampi(y, a.Hc) - cy0) * CPatch<T, RH, NT>::PW;  // uniform
+  const T c00 = spatch[r + pc.c0], c10 = spatch[r + pc.c1];
+  return (double)c00 + (double)(T)(c10 - c00) * pc.tx;
+}
+
+// y weight of fine row gy: its position minus the clamped coarse floor
+__device__ __forceinline__ double prolong_ty(int gy, int Hc) {
+  return ((gy + 0.5) * 0.5 - 0.5) - (double)clampi((gy - 1) >> 1, Hc);
+}
+
+// The prolonged corrections of fine rows gya and gya + 1 (gya uniform) at the
+// lane's column.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void prolong_pair(const SmoothArgs<T>& a, const T* spatch,
+                                             const PCol& pc, int cy0, int gya, T& ca, T& cb) {
+  const int r0 = (gya - 1) >> 1;
+  const double l0 = prolong_xlerp<T, RH, NT>(a, spatch, pc, cy0, r0);
+  const double l1 = prolong_xlerp<T, RH, NT>(a, spatch, pc, cy0, r0 + 1);
+  const double tya = prolong_ty(gya, a.Hc), tyb = prolong_ty(gya + 1, a.Hc);
+  ca = (T)(l0 + (l1 - l0) * tya);
+  if (gya & 1) {  // row gya + 1 samples coarse rows r0 and r0 + 1 too
+    cb = (T)(l0 + (l1 - l0) * tyb);
+  } else {
+    const double l2 = prolong_xlerp<T, RH, NT>(a, spatch, pc, cy0, r0 + 2);
+    cb = (T)(l1 + (l2 - l1) * tyb);
+  }
+}
+
+// Raw loads of the tile's rows for the lane's pairs h0 .. h0+KH-1 (clamped
+// into the grid, unconditional: a lane issues them back to back).
+template <typename T, int RH, int NT, int KH>
+__device__ __forceinline__ void rtile_issue(const SmoothArgs<T>& a, int bx, int by, int h0,
+                                            T (&pv)[KH][2], T (&rv)[KH][2],
+                                            uint32_t (&fv)[KH][2]) {
+  using RT = RTile<T, RH, NT>;
+  const int R = a.R;
+  const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
+  const int lane = threadIdx.x & 63, wave = wave_id();
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const unsigned gxc = (unsigned)clampi(gx, a.W), gxo = gxc * (unsigned)sizeof(T);
+#pragma unroll
+  for (int k = 0; k < KH; ++k)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int gy = gy0 + 2 * (wave + (h0 + k) * RT::NW) + q;
+      const size_t row = (size_t)clampi(gy, a.H) * a.W;
+      fv[k][q] = row_at(a.flags + row, gxc);
+      rv[k][q] = row_at(a.rhs + row, gxo);
+      pv[k][q] = a.zero_init ? (T)0 : row_at(a.p_in + row, gxo);
+    }
+}
+
+// Apply the prolonged correction (spatch: the staged coarse patch, or null),
+// zero what lies outside the grid, and place p and the flags in LDS and the
+// rhs and flags in the lane's registers.
+template <typename T, int RH, int NT, int KH>
+__device__ __forceinline__ void rtile_commit(const SmoothArgs<T>& a, int bx, int by, int h0,
+                                             T (&pv)[KH][2], const T (&rv)[KH][2],
+                                             const uint32_t (&fv)[KH][2], T* sp, uint8_t* sf,
+                                             RTile<T, RH, NT>& t, const T* spatch,
+                                             const PCol& pc) {
+  using RT = RTile<T, RH, NT>;
+  constexpr uint32_t F_ALL = F_FLUID | F_LFLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
+  const int R = a.R;
+  const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
+  const int lane = threadIdx.x & 63, wave = wave_id();
+  const int px = lane & 1;
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const bool colin = gx >= 0 && gx < a.W;
+  if (spatch != nullptr) {  // the prolonged correction, on fluid cells
+    const int cy0 = (gy0 - 1) >> 1;
+#pragma unroll
+    for (int k = 0; k < KH; ++k) {
+      T ca, cb;
+      prolong_pair<T, RH, NT>(a, spatch, pc, cy0, gy0 + 2 * (wave + (h0 + k) * RT::NW), ca, cb);
+      pv[k][0] = (fv[k][0] & F_LFLUID) ? pv[k][0] + ca : pv[k][0];
+      pv[k][1] = (fv[k][1] & F_LFLUID) ? pv[k][1] + cb : pv[k][1];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    uint32_t f[2];
+    T r[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int ly = 2 * (wave + (h0 + k) * RT::NW) + q;
+      const int gy = gy0 + ly;
+      const bool in = colin && gy >= 0 && gy < a.H;
+      const int o = ly * RT::RW + lane;
+      f[q] = in ? fv[k][q] : 0u;
+      r[q] = in ? rv[k][q] : (T)0;
+      sp[o] = in ? pv[k][q] : (T)0;
+      sf[o] = (uint8_t)f[q];
+    }
+    // row 2m + q has colour (q + x) & 1
+    const int kk = h0 + k;
+    t.rb[0][kk] = px ? r[1] : r[0];
+    t.rb[1][kk] = px ? r[0] : r[1];
+    const uint32_t f0 = px ? f[1] : f[0], f1 = px ? f[0] : f[1];
+    if (__ballot((f0 & F_ALL) == F_ALL) == ~0ull) t.open[0] |= 1u << kk;
+    if (__ballot((f1 & F_ALL) == F_ALL) == ~0ull) t.open[1] |= 1u << kk;
+  }
+}
+
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_reset(RTile<T, RH, NT>& t) {
+  t.open[0] = t.open[1] = 0u;
+}
+
+// OPEN: every pair of the lane's colour-C cells is open in every lane (no
+// selects), decided once per pass rather than per cell, so the pass is one
+// straight run of code.  A cell's reads are its own value and cells of the
+// other colour, which the pass does not write, so a pair's new values are
+// stored before the next pair reads.  Every cell of the region is updated
+// (SPad); the fluid flag alone decides the store.
+template <int C, bool OPEN, bool FM, typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                              T* sp, const uint8_t* sf) {
+  using RT = RTile<T, RH, NT>;
+  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
+  const int lane = threadIdx.x & 63, wave = wave_id();
+  const int par = (lane + C) & 1;
+  const int o0 = (2 * wave + par) * RW + lane;  // the lane's cell of pair 0
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int o = o0 + 2 * k * NW * RW;
+    const T ce = sp[o];
+    const T lv = sp[o - 1], rv = sp[o + 1], dv = sp[o - RW], uv = sp[o + RW];
+    T ns;
+    bool fl;
+    if (OPEN) {
+      ns = lv + rv + dv + uv;
+      fl = true;
+    } else {
+      const uint32_t f = sf[o];
+      const T l = (f & F_LEFT) ? lv : ce;
+      const T r = (f & F_RIGHT) ? rv : ce;
+      const T d = (f & F_DOWN) ? dv : ce;
+      const T u = (f & F_UP) ? uv : ce;
+      ns = l + r + d + u;
+      fl = (f & F_FLUID) != 0u;
+    }
+    T nv;
+    if constexpr (FM && sizeof(T) == 4)
+      nv = relax_f(ce, ns, t.rb[C][k], (float)a.alpha, (float)a.omega);
+    else
+      nv = (T)relax_value(0, ce, ns, t.rb[C][k], a.alpha, a.omega);
+    if (fl) sp[o] = nv;
+  }
+}
+
+template <int C, bool FM, typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_sweep(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                            T* sp, const uint8_t* sf) {
+  constexpr uint32_t ALL = (1u << RTile<T, RH, NT>::K) - 1u;
+  if ((t.open[C] & ALL) == ALL)  // wave-uniform
+    rtile_sweep_k<C, true, FM>(a, t, sp, sf);
+  else
+    rtile_sweep_k<C, false, FM>(a, t, sp, sf);
+}
+
+template <bool FM, typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_passes_m(const SmoothArgs<T>& a,
+                                               const RTile<T, RH, NT>& t, T* sp,
+                                               const uint8_t* sf) {
+  for (int pass = 0; pass < a.npass; ++pass) {
+    if (((a.first_colour + pass) & 1) == 0)
+      rtile_sweep<0, FM>(a, t, sp, sf);
+    else
+      rtile_sweep<1, FM>(a, t, sp, sf);
+    __syncthreads();
+  }
+}
+
+// npass colour passes over the whole region (the exact window shrinks by one
+// cell per pass).
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_passes(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                             T* sp, const uint8_t* sf) {
+  if (sizeof(T) == 4 && a.fast_math)
+    rtile_passes_m<true>(a, t, sp, sf);
+  else
+    rtile_passes_m<false>(a, t, sp, sf);
+}
+
+// Write the tile interior of p; then the residuals of the lane's own cells
+// in the output window feed the restriction (through the p array) and the
+// residual norm.
+template <typename T, int RH, int NT>
+__device__ __forceinline__ void rtile_epilogue(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
+                                               int bx, int by, int part, T* sp,
+                                               const uint8_t* sf, double* red) {
+  using RT = RTile<T, RH, NT>;
+  constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
+  const int R = a.R;
+  const int TX = RW - 2 * R, TY = RH - 2 * R;
+  const int H = a.H, W = a.W;
+  const int lane = threadIdx.x & 63, wave = wave_id();
+  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
+  const bool colin = gx >= 0 && gx < W;
+  const bool xout = lane >= R && lane < R + TX && colin;
+  const unsigned gxo = (unsigned)(colin ? gx : 0) * (unsigned)sizeof(T);
+  const int lyend = min(R + TY, H - gy0);  // uniform: output rows end
+  for (int ly = R + wave; ly < lyend; ly += NW)
+    if (xout) row_at(a.p_out + (size_t)(gy0 + ly) * W, gxo) = sp[ly * RW + lane];
+  const bool want_norm = a.norm_partial != nullptr;
+  if (a.coarse_rhs == nullptr && !want_norm) return;
+
+  T res[2][K];
+  double acc = 0.0;
+  constexpr uint32_t ALL = (1u << K) - 1u;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int par = (lane + c) & 1;
+    // as in rtile_sweep_k: one wave-uniform decision for the colour's pairs
+    const bool open_all = (t.open[c] & ALL) == ALL;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      // every cell of the region (SPad); only the output window's are used
+      const int ly = 2 * (wave + k * NW) + par;
+      const int o = ly * RW + lane;
+      const T ce = sp[o];
+      const T lv = sp[o - 1], rv = sp[o + 1], dv = sp[o - RW], uv = sp[o + RW];
+      T ns;
+      bool fl;
+      if (open_all) {
+        ns = lv + rv + dv + uv;
+        fl = true;
+      } else {
+        const uint32_t f = sf[o];
+        const T l = (f & F_LEFT) ? lv : ce;
+        const T r = (f & F_RIGHT) ? rv : ce;
+        const T d = (f & F_DOWN) ? dv : ce;
+        const T u = (f & F_UP) ? uv : ce;
+        ns = l + r + d + u;
+        fl = (f & F_LFLUID) != 0u;
+      }
+      const double rr = (sizeof(T) == 4 && a.fast_math)
+                            ? (double)residual_f((float)ce, (float)ns, (float)t.rb[c][k],
+                                                 (float)a.alpha)
+                            : residual_value(ce, ns, t.rb[c][k], a.alpha);
+      res[c][k] = (T)rr;
+      if (want_norm) {
+        const bool win = xout && ly >= R && ly < lyend;
+        acc += (win && fl) ? rr * rr : 0.0;
+      }
+    }
+  }
+
+  if (a.coarse_rhs != nullptr) {
+    __syncthreads();  // every lane has read p
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int par = (lane + c) & 1;
+#pragma unroll
+      for (int k = 0; k < K; ++k) sp[(2 * (wave + k * NW) + par) * RW + lane] = res[c][k];
+    }
+    __syncthreads();
+    const int CX = TX / 2, CY = TY / 2;
+    const int Wc = W / 2, Hc = H / 2;
+    const int cx = lane & 31;
+    const int Ic = bx * CX + cx;
+    const bool cxin = cx < CX && Ic < Wc;
+    const unsigned co = (unsigned)(cxin ? Ic : 0) * (unsigned)sizeof(T);
+    // one coarse row per half-wavefront: rows cyb and cyb + 1 of a wave
+    const int cyend = min(CY, Hc - by * CY);  // uniform
+    for (int cyb = 2 * wave; cyb < cyend; cyb += 2 * NW) {
+      const int cy = cyb + (lane >> 5);
+      if (cy >= cyend || !cxin) continue;
+      // the children's residuals summed in float64 over the fluid children
+      // in the (dj, di) order of smooth_tile64_body
+      double total = 0.0;
+      int count = 0;
+#pragma unroll
+      for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+        for (int di = 0; di < 2; ++di) {
+          const int o = (R + 2 * cy + dj) * RW + R + 2 * cx + di;
+          const bool fl = sf[o] & F_LFLUID;
+          const T rv = sp[o];
+          total += fl ? (double)rv : 0.0;
+          count += fl ? 1 : 0;
+        }
+      // total / count, exact scalings for 4, 2 and 1 children (the quotient
+      // by a power of two is the product by its reciprocal, both correctly
+      // rounded); no children gives 0 as before
+      const double mean_d = count == 3 ? total / 3.0
+                                       : total * (count == 4 ? 0.25 : (count == 2 ? 0.5 : 1.0));
+      const T mean = (T)mean_d;
+      row_at(a.coarse_rhs + (size_t)(by * CY + cy) * Wc, co) = (T)((double)mean / a.coarse_div);
+    }
+  }
+
+  if (want_norm) {
+    const double tot = block_sum<NT>(acc, red);
+    if (threadIdx.x == 0) a.norm_partial[part] = tot;
+  }
 }
 
 // The prolonged correction added to the lane's fluid cells in LDS (the p of
 // the region is staged; the patch is in LDS; a barrier follows).
 template <typename T, int RH, int NT>
-__device__ __forceinline__ void rtile_prolong(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                              int bx, int by, T* sp, const T* spatch) {
+__device__ __forceinline__ void rtile_prolong(const SmoothArgs<T>& a, int bx, int by, T* sp,
+                                              const uint8_t* sf, const T* spatch) {
   using RT = RTile<T, RH, NT>;
   constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
   const int R = a.R;
-  const int TX = RW - 2 * R, TY = RH - 2 * R;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gx = bx * TX - R + lane, gy0 = by * TY - R;
-  const int px = lane & 1;
+  const int TY = RH - 2 * R;
+  const int lane = threadIdx.x & 63, wave = wave_id();
+  const int gy0 = by * TY - R;
+  const int cy0 = (gy0 - 1) >> 1;
+  const PCol pc = prolong_col<T, RH, NT>(a, bx);
 #pragma unroll
-  for (int k = 0; k < K; ++k)
+  for (int k = 0; k < K; ++k) {
+    T corr[2];
+    prolong_pair<T, RH, NT>(a, spatch, pc, cy0, gy0 + 2 * (wave + k * NW), corr[0], corr[1]);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int ly = 2 * (wave + k * NW) + q;
-      const T corr = prolong_at<T, RH, NT>(a, spatch, bx, by, gx, gy0 + ly);
-      // colour of row 2m + q is (q + x) & 1; select the word, not the array
-      const uint32_t fwd = ((q + px) & 1) ? t.fw[1][k >> 2] : t.fw[0][k >> 2];
-      const bool fl = ((fwd >> (8 * (k & 3))) & F_LFLUID) != 0u;
       const int o = ly * RW + lane;
-      sp[o] = fl ? sp[o] + corr : sp[o];
+      sp[o] = (sf[o] & F_LFLUID) ? sp[o] + corr[q] : sp[o];
     }
+  }
 }
 
 template <typename T, int RH, int NT>
@@ -740,7 +789,8 @@ __device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int 
                                                     int part) {
   using RT = RTile<T, RH, NT>;
   using CP = CPatch<T, RH, NT>;
-  __shared__ T sp[RH * RT::RW];
+  __shared__ T sp_[SPad<T, RH>::N];
+  T* sp = sp_ + SPad<T, RH>::OFF;
   __shared__ uint8_t sf[RH * RT::RW];
   __shared__ double red[RT::NW];
   __shared__ T spatch[CP::N];
@@ -749,11 +799,15 @@ __device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int 
   rtile_reset(t);
   const bool prolong = a.coarse_e != nullptr;
   T cv[CP::PER];
-  if (prolong) cpatch_issue<T, RH, NT>(a, bx, by, cv);  // overlaps the tile's loads
-  // in two halves of the lane's pairs, to bound the registers of loads in
-  // flight
+  PCol pc{};
+  if (prolong) {
+    cpatch_issue<T, RH, NT>(a, bx, by, cv);  // overlaps the tile's loads
+    pc = prolong_col<T, RH, NT>(a, bx);
+  }
+  // in two halves of the lane's pairs (quarters from eight pairs on), to
+  // bound the registers of loads in flight
   constexpr int K = RT::K;
-  constexpr int KH = (K % 2 == 0) ? K / 2 : K;
+  constexpr int KH = K >= 8 ? K / 4 : ((K % 2 == 0) ? K / 2 : K);
 #pragma unroll
   for (int h0 = 0; h0 < K; h0 += KH) {
     T pv[KH][2], rv[KH][2];
@@ -764,12 +818,12 @@ __device__ __forceinline__ void smooth_tile64r_body(const SmoothArgs<T>& a, int 
       __syncthreads();
     }
     rtile_commit<T, RH, NT, KH>(a, bx, by, h0, pv, rv, fv, sp, sf, t,
-                                prolong ? spatch : nullptr);
+                                prolong ? spatch : nullptr, pc);
   }
   t.open[0] = __builtin_amdgcn_readfirstlane(t.open[0]);
   t.open[1] = __builtin_amdgcn_readfirstlane(t.open[1]);
   __syncthreads();
-  rtile_passes(a, t, sp);
+  rtile_passes(a, t, sp, sf);
   rtile_epilogue(a, t, bx, by, part, sp, sf, red);
 }
 
@@ -785,73 +839,74 @@ __device__ __forceinline__ void rtile_dma_p(const SmoothArgs<T>& a, int bx, int 
   constexpr int RW = 64, NW = NT / 64;
   constexpr int EPD = (int)sizeof(T) / 4;  // dwords per value
   constexpr int CPI = 64 / EPD;            // values per instruction
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   const int R = a.R, TX = RW - 2 * R, TY = RH - 2 * R;
   const int gx0 = bx * TX - R, gy0 = by * TY - R;
 #pragma unroll
-  for (int k = 0; k < RH / NW; ++k) {
-    const int ly = wave + k * NW;
-    int gy = gy0 + ly;
-    gy = gy < 0 ? 0 : (gy >= a.H ? a.H - 1 : gy);
+  for (int h = 0; h < EPD; ++h) {
+    const unsigned off =
+        (unsigned)clampi(gx0 + h * CPI + lane / EPD, a.W) * (unsigned)sizeof(T) + 4u * (lane % EPD);
 #pragma unroll
-    for (int h = 0; h < EPD; ++h) {
-      int gx = gx0 + h * CPI + lane / EPD;
-      gx = gx < 0 ? 0 : (gx >= a.W ? a.W - 1 : gx);
-      const char* src =
-          reinterpret_cast<const char*>(a.p_in + (size_t)gy * a.W + gx) + 4 * (lane % EPD);
+    for (int k = 0; k < RH / NW; ++k) {
+      const int ly = wave + k * NW;
+      const T* row = a.p_in + (size_t)clampi(gy0 + ly, a.H) * a.W;
       __builtin_amdgcn_global_load_lds(
-          src, (__attribute__((address_space(3))) void*)(sp + ly * RW + h * CPI), 4, 0, 0);
+          &row_at(row, off), (__attribute__((address_space(3))) void*)(sp + ly * RW + h * CPI),
+          4, 0, 0);
     }
   }
 }
 
-// rhs and flags of the lane's cells (all pairs at once) into its registers,
-// the flags also into LDS for the restriction.
+// rhs and flags of the lane's cells into its registers (all pairs at once, or
+// in groups of four pairs from eight pairs on, to bound the registers of loads
+// in flight), the flags also into LDS for the restriction.
 template <typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_stage_rf(const SmoothArgs<T>& a, int bx, int by,
                                                uint8_t* sf, RTile<T, RH, NT>& t) {
   using RT = RTile<T, RH, NT>;
   constexpr int K = RT::K;
+  constexpr int KG = K >= 8 ? 4 : K;
   constexpr uint32_t F_ALL = F_FLUID | F_LFLUID | F_LEFT | F_RIGHT | F_DOWN | F_UP;
   const int R = a.R;
   const int TX = RT::RW - 2 * R, TY = RH - 2 * R;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   const int px = lane & 1;
   const int gx = bx * TX - R + lane, gy0 = by * TY - R;
   const bool colin = gx >= 0 && gx < a.W;
-  const int gxc = gx < 0 ? 0 : (gx >= a.W ? a.W - 1 : gx);
-  T rv[K][2];
-  uint32_t fv[K][2];
+  const unsigned gxc = (unsigned)clampi(gx, a.W), gxo = gxc * (unsigned)sizeof(T);
 #pragma unroll
-  for (int k = 0; k < K; ++k)
+  for (int k0 = 0; k0 < K; k0 += KG) {
+    T rv[KG][2];
+    uint32_t fv[KG][2];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int gy = gy0 + 2 * (wave + k * RT::NW) + q;
-      const int gyc = gy < 0 ? 0 : (gy >= a.H ? a.H - 1 : gy);
-      const size_t g = (size_t)gyc * a.W + gxc;
-      fv[k][q] = a.flags[g];
-      rv[k][q] = a.rhs[g];
+    for (int j = 0; j < KG; ++j)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int gy = gy0 + 2 * (wave + (k0 + j) * RT::NW) + q;
+        const size_t row = (size_t)clampi(gy, a.H) * a.W;
+        fv[j][q] = row_at(a.flags + row, gxc);
+        rv[j][q] = row_at(a.rhs + row, gxo);
+      }
+#pragma unroll
+    for (int j = 0; j < KG; ++j) {
+      const int k = k0 + j;
+      uint32_t f[2];
+      T r[2];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int ly = 2 * (wave + k * RT::NW) + q;
+        const int gy = gy0 + ly;
+        const bool in = colin && gy >= 0 && gy < a.H;
+        f[q] = in ? fv[j][q] : 0u;
+        r[q] = in ? rv[j][q] : (T)0;
+        sf[ly * RT::RW + lane] = (uint8_t)f[q];
+      }
+      t.rb[0][k] = px ? r[1] : r[0];
+      t.rb[1][k] = px ? r[0] : r[1];
+      const uint32_t f0 = px ? f[1] : f[0], f1 = px ? f[0] : f[1];
+      if (__ballot((f0 & F_ALL) == F_ALL) == ~0ull) t.open[0] |= 1u << k;
+      if (__ballot((f1 & F_ALL) == F_ALL) == ~0ull) t.open[1] |= 1u << k;
     }
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    uint32_t f[2];
-    T r[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int ly = 2 * (wave + k * RT::NW) + q;
-      const int gy = gy0 + ly;
-      const bool in = colin && gy >= 0 && gy < a.H;
-      f[q] = in ? fv[k][q] : 0u;
-      r[q] = in ? rv[k][q] : (T)0;
-      sf[ly * RT::RW + lane] = (uint8_t)f[q];
-    }
-    t.rb[0][k] = px ? r[1] : r[0];
-    t.rb[1][k] = px ? r[0] : r[1];
-    const uint32_t f0 = px ? f[1] : f[0], f1 = px ? f[0] : f[1];
-    t.fw[0][k >> 2] |= f0 << (8 * (k & 3));
-    t.fw[1][k >> 2] |= f1 << (8 * (k & 3));
-    if (__ballot((f0 & F_ALL) == F_ALL) == ~0ull) t.open[0] |= 1u << k;
-    if (__ballot((f1 & F_ALL) == F_ALL) == ~0ull) t.open[1] |= 1u << k;
   }
 }
 
@@ -861,12 +916,13 @@ __device__ __forceinline__ void smooth_tile64d_body(const SmoothArgs<T>& a, int 
   using RT = RTile<T, RH, NT>;
   using CP = CPatch<T, RH, NT>;
   constexpr int RW = RT::RW, NW = RT::NW;
-  __shared__ T sp[RH * RW];
+  __shared__ T sp_[SPad<T, RH>::N];
+  T* sp = sp_ + SPad<T, RH>::OFF;
   __shared__ uint8_t sf[RH * RW];
   __shared__ double red[NW];
   __shared__ T spatch[CP::N];
   if (a.gate != nullptr && *a.gate == 0) return;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = wave_id();
   RT t;
   rtile_reset(t);
   if (a.zero_init) {
@@ -885,21 +941,22 @@ __device__ __forceinline__ void smooth_tile64d_body(const SmoothArgs<T>& a, int 
   __builtin_amdgcn_s_waitcnt(0);  // the LDS-DMA writes have landed
   __syncthreads();
   if (prolong) {
-    rtile_prolong<T, RH, NT>(a, t, bx, by, sp, spatch);
+    rtile_prolong<T, RH, NT>(a, bx, by, sp, sf, spatch);
     __syncthreads();
   }
-  rtile_passes(a, t, sp);
+  rtile_passes(a, t, sp, sf);
   rtile_epilogue(a, t, bx, by, part, sp, sf, red);
 }
 
 // waves per SIMD the register allocation must allow: 4 when a float64 lane
 // holds six or more row pairs (more would spill); 8 for the float32 64-row
-// tiles (64 VGPRs: four 512-thread workgroups per CU instead of three, three
-// dwords spilled outside the pass loops; 24.6 -> 24.2 us at 2048^2); 5
-// otherwise
+// tiles (four 512-thread workgroups per CU); 6 for float64 lanes of four
+// pairs (three 512-thread workgroups per CU, as the LDS of the 64-row tile
+// allows); 5 otherwise
 template <typename T, int RH, int NT>
 constexpr int r_waves() {
-  return (sizeof(T) == 8 && RH / (2 * (NT / 64)) >= 6) ? 4 : (sizeof(T) == 4 && RH == 64 ? 8 : 5);
+  return sizeof(T) == 8 ? (RH / (2 * (NT / 64)) >= 6 ? 4 : (RH / (2 * (NT / 64)) == 4 ? 6 : 5))
+                        : (RH == 64 ? 8 : 5);
 }
 
 template <typename T, int RH, int NT>
@@ -916,15 +973,22 @@ __global__ __launch_bounds__(NT, (r_waves<T, RH, NT>())) void k_smooth_tile64_r(
   smooth_tile64r_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
 }
 
+// the LDS-DMA variants: six waves per SIMD, four for a float64 lane with six
+// or more row pairs
 template <typename T, int RH, int NT>
-__global__ __launch_bounds__(NT, 6) void k_smooth_fine_d(const SmoothArgs<T> a) {
+constexpr int d_waves() {
+  return (sizeof(T) == 8 && RH / (2 * (NT / 64)) >= 6) ? 4 : 6;
+}
+
+template <typename T, int RH, int NT>
+__global__ __launch_bounds__(NT, (d_waves<T, RH, NT>())) void k_smooth_fine_d(const SmoothArgs<T> a) {
   int bx, by;
   xcd_tile(bx, by);
   smooth_tile64d_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
 }
 
 template <typename T, int RH, int NT>
-__global__ __launch_bounds__(NT, 6) void k_smooth_tile64_d(const SmoothArgs<T> a) {
+__global__ __launch_bounds__(NT, (d_waves<T, RH, NT>())) void k_smooth_tile64_d(const SmoothArgs<T> a) {
   int bx, by;
   xcd_tile(bx, by);
   smooth_tile64d_body<T, RH, NT>(a, bx, by, by * gridDim.x + bx);
