@@ -719,6 +719,11 @@ def main():
         log(f"cpu baseline: {cpu}")
 
     extra = {}
+    from cfd_amd import tuning
+
+    # the 2048^2 autotuning chose variants for that grid; the other workloads
+    # run with the library's defaults
+    tuning.restore_defaults()
     if not args.no_small_configs:
         log("128^2 Jacobi and 512^2 RB-SOR plumes")
         extra.update(bench_small_configs())

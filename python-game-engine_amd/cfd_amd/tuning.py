@@ -33,6 +33,24 @@ CANDIDATES = {
 }
 
 _CACHE: dict = {}
+# the variants the process started with, per (dtype code, level class)
+_DEFAULTS: dict = {}
+
+
+def _remember_defaults() -> None:
+    if _DEFAULTS:
+        return
+    for code in (_lib.SMK_F64, _lib.SMK_F32):
+        for which in CANDIDATES:
+            _DEFAULTS[(code, which)] = _current(code, which)
+
+
+def restore_defaults() -> None:
+    """Put back the tile variants the process started with.  The variants are
+    process-wide per dtype, so a choice :func:`autotune` made for one grid
+    otherwise stays in force for every later solve of that dtype."""
+    for (code, which), name in _DEFAULTS.items():
+        _lib.set_smoother_variant(code, which, name)
 
 
 def _names(which):
@@ -107,6 +125,7 @@ def autotune(sim, dt=None, *, steps: int = 2, repeats: int = 2, classes=tuple(CA
     from dataclasses import replace
 
     dt = sim.dt if dt is None else dt
+    _remember_defaults()
     code = _variant_dtype(sim)
     key = (sim.height, sim.width, str(sim.dtype), _solve_key(sim), tuple(classes),
            tuple(sweeps) if sweeps else None)

@@ -128,7 +128,10 @@ __global__ void k_advect_scalar(const T* __restrict__ src, T* __restrict__ dst, 
   const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
   const size_t o = (size_t)j * W + i;
-  if (!fluid[o]) { dst[o] = (T)0; return; }
+  // the mask is read with the velocity samples, not before them: every cell
+  // is traced (all sample indices are clamped into the fields) and the store
+  // selects, so the loads of a lane start together
+  const bool fl = fluid[o] != 0;
   const double px = i + 0.5, py = j + 0.5;
   const double uu = sample_field(u, uh, uw, px, py - 0.5);
   const double vv = sample_field(v, vh, vw, px - 0.5, py);
@@ -136,7 +139,7 @@ __global__ void k_advect_scalar(const T* __restrict__ src, T* __restrict__ dst, 
   const double y = py - (vv * dt * inv);
   T r = (T)sample_field(src, H, W, x - 0.5, y - 0.5);
   if (apply_scale) r = r * (T)scale;
-  dst[o] = r;
+  dst[o] = fl ? r : (T)0;
 }
 
 template <typename T>
@@ -145,13 +148,15 @@ __device__ __forceinline__ void advect_u_at(const T* __restrict__ us, T* __restr
                                             const uint8_t* __restrict__ fmu, double dt,
                                             double inv, int j, int i) {
   const size_t o = (size_t)j * uw + i;
-  if (!fmu[o]) { ud[o] = (T)0; return; }
+  // as in k_advect_scalar: every face is traced, the mask selects the store
+  const bool open = fmu[o] != 0;
   const double px = (double)i, py = j + 0.5;
   const double uu = (double)us[o];
   const double vv = sample_field(v, vh, vw, px - 0.5, py);
   const double x = px - (uu * dt * inv);
   const double y = py - (vv * dt * inv);
-  ud[o] = (T)sample_field(us, uh, uw, x, y - 0.5);
+  const T r = (T)sample_field(us, uh, uw, x, y - 0.5);
+  ud[o] = open ? r : (T)0;
 }
 
 template <typename T>
@@ -160,13 +165,14 @@ __device__ __forceinline__ void advect_v_at(const T* __restrict__ vs, T* __restr
                                             const uint8_t* __restrict__ fmv, double dt,
                                             double inv, int j, int i) {
   const size_t o = (size_t)j * vw + i;
-  if (!fmv[o]) { vd[o] = (T)0; return; }
+  const bool open = fmv[o] != 0;
   const double px = i + 0.5, py = (double)j;
   const double uu = sample_field(u, uh, uw, px, py - 0.5);
   const double vv = (double)vs[o];
   const double x = px - (uu * dt * inv);
   const double y = py - (vv * dt * inv);
-  vd[o] = (T)sample_field(vs, vh, vw, x - 0.5, y);
+  const T r = (T)sample_field(vs, vh, vw, x - 0.5, y);
+  vd[o] = open ? r : (T)0;
 }
 
 template <typename T>
@@ -290,38 +296,38 @@ __global__ void k_pressure_gradient(T* __restrict__ u, T* __restrict__ v,
   int bx_, by_;
   xcd_tile(bx_, by_);
   const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
+  // every operand is loaded before any is tested (indices clamped into the
+  // arrays), so a lane's loads start together; masks and walls select
   if (j < H && i <= W) {
     const size_t o = (size_t)j * (W + 1) + i;
-    T val;
-    if (fmu[o]) {
-      const int ir = i < W ? i : W - 1;      // open face past the end: clamped
-      const int il = i - 1 < 0 ? W - 1 : i - 1;  // index -1 wraps (Python)
-      const T dp = p[(size_t)j * W + ir] - p[(size_t)j * W + il];
-      val = (T)((double)u[o] - scale * (double)dp);
-    } else {
-      val = (T)0;
-    }
+    const int ir = i < W ? i : W - 1;          // open face past the end: clamped
+    const int il = i - 1 < 0 ? W - 1 : i - 1;  // index -1 wraps (Python)
+    const bool open = fmu[o] != 0;
+    const T uo = u[o];
+    const T dp = p[(size_t)j * W + ir] - p[(size_t)j * W + il];
+    T val = open ? (T)((double)uo - scale * (double)dp) : (T)0;
     if (solid != nullptr) {
-      const bool ls = i > 0 ? solid[(size_t)j * W + i - 1] == 1 : false;
-      const bool rs = i < W ? solid[(size_t)j * W + i] == 1 : false;
+      const uint8_t sl = solid[(size_t)j * W + (i > 0 ? i - 1 : 0)];
+      const uint8_t sr = solid[(size_t)j * W + (i < W ? i : W - 1)];
+      const bool ls = i > 0 && sl == 1;
+      const bool rs = i < W && sr == 1;
       if (ls || rs) val = (T)0;
     }
     u[o] = val;
   }
   if (j <= H && i < W) {
     const size_t o = (size_t)j * W + i;
-    T val;
-    if (fmv[o]) {
-      const int jt = j < H ? j : H - 1;
-      const int jb = j - 1 < 0 ? H - 1 : j - 1;
-      const T dp = p[(size_t)jt * W + i] - p[(size_t)jb * W + i];
-      val = (T)((double)v[o] - scale * (double)dp);
-    } else {
-      val = (T)0;
-    }
+    const int jt = j < H ? j : H - 1;
+    const int jb = j - 1 < 0 ? H - 1 : j - 1;
+    const bool open = fmv[o] != 0;
+    const T vo = v[o];
+    const T dp = p[(size_t)jt * W + i] - p[(size_t)jb * W + i];
+    T val = open ? (T)((double)vo - scale * (double)dp) : (T)0;
     if (solid != nullptr) {
-      const bool ts = j > 0 ? solid[(size_t)(j - 1) * W + i] == 1 : false;
-      const bool bs = j < H ? solid[(size_t)j * W + i] == 1 : false;
+      const uint8_t st = solid[(size_t)(j > 0 ? j - 1 : 0) * W + i];
+      const uint8_t sb = solid[(size_t)(j < H ? j : H - 1) * W + i];
+      const bool ts = j > 0 && st == 1;
+      const bool bs = j < H && sb == 1;
       if (ts || bs) val = (T)0;
     }
     v[o] = val;

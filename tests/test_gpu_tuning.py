@@ -61,3 +61,23 @@ def test_autotune_sweeps(keep_variants):
     assert sim.solver_params.smoother_iterations == res["chosen"]["smoother_iterations"]
     sim.substep(sc.dt)
     assert sim.last_solve_info["converged"] == 1
+
+
+def test_restore_defaults(keep_variants):
+    """restore_defaults() puts back the variants the process started with, which
+    autotune() changes process-wide."""
+    def snapshot():
+        return {(c, w): _lib.lib().smk_get_smoother_variant(c, _lib._WHICH[w])
+                for c in (_lib.SMK_F64, _lib.SMK_F32) for w in tuning.CANDIDATES}
+
+    start = snapshot()
+    sc = scenes.plume_2048_mg(N=128, steps=2)
+    sim = scenes.make_simulation(sc, dtype="float64", solver_overrides=MIXED)
+    sim.substep(sc.dt)
+    tuning.autotune(sim, steps=1, repeats=1, use_cache=False)
+    for code in (_lib.SMK_F64, _lib.SMK_F32):
+        _lib.set_smoother_variant(code, "fine", "lds32")
+        _lib.set_smoother_variant(code, "small", "nt256")
+    assert snapshot() != start
+    tuning.restore_defaults()
+    assert snapshot() == start
