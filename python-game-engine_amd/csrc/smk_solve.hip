@@ -379,11 +379,11 @@ struct RTile {
 // p is staged in an LDS array of RH + 2 rows: the region's rows 0 .. RH-1
 // and one row of padding above and below, so that every cell of the region
 // can read its four neighbours in range.  The passes then update every cell
-// of the region, not only those of the shrinking exact window: after pass k
+// of a row pair, not only those of the shrinking exact window: after pass k
 // the cells at distance >= k from the region's edge are exact whatever the
 // others hold (an exact cell reads only cells at distance >= k - 1 of the
 // other colour), so the extra updates change nothing that is written out,
-// and no lane or pass needs a window test.
+// and no lane needs a window test.
 template <typename T, int RH>
 struct SPad {
   static constexpr int N = (RH + 2) * 64;  // elements of the LDS array
@@ -572,11 +572,14 @@ __device__ __forceinline__ void rtile_reset(RTile<T, RH, NT>& t) {
 // selects), decided once per pass rather than per cell, so the pass is one
 // straight run of code.  A cell's reads are its own value and cells of the
 // other colour, which the pass does not write, so a pair's new values are
-// stored before the next pair reads.  Every cell of the region is updated
-// (SPad); the fluid flag alone decides the store.
-template <int C, bool OPEN, bool FM, typename T, int RH, int NT>
+// stored before the next pair reads.  Every cell of a pair is updated
+// (SPad), the fluid flag alone deciding the store.  With SKIP (launches of
+// many passes, where the exact window shrinks far) a pair whose two rows
+// both lie outside the pass's exact window (rows lo .. RH-lo-1) is skipped
+// by the whole wavefront; the test costs more than it saves at 4 passes.
+template <int C, bool OPEN, bool FM, bool SKIP, typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                              T* sp, const uint8_t* sf) {
+                                              T* sp, const uint8_t* sf, int lo) {
   using RT = RTile<T, RH, NT>;
   constexpr int K = RT::K, RW = RT::RW, NW = RT::NW;
   const int lane = threadIdx.x & 63, wave = wave_id();
@@ -584,6 +587,8 @@ __device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTil
   const int o0 = (2 * wave + par) * RW + lane;  // the lane's cell of pair 0
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    const int lyp = 2 * (wave + k * NW);  // uniform: the pair's first row
+    if (SKIP && (lyp + 1 < lo || lyp >= RH - lo)) continue;
     const int o = o0 + 2 * k * NW * RW;
     const T ce = sp[o];
     const T lv = sp[o - 1], rv = sp[o + 1], dv = sp[o - RW], uv = sp[o + RW];
@@ -610,38 +615,50 @@ __device__ __forceinline__ void rtile_sweep_k(const SmoothArgs<T>& a, const RTil
   }
 }
 
-template <int C, bool FM, typename T, int RH, int NT>
+template <int C, bool FM, bool SKIP, typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_sweep(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
-                                            T* sp, const uint8_t* sf) {
+                                            T* sp, const uint8_t* sf, int lo) {
   constexpr uint32_t ALL = (1u << RTile<T, RH, NT>::K) - 1u;
   if ((t.open[C] & ALL) == ALL)  // wave-uniform
-    rtile_sweep_k<C, true, FM>(a, t, sp, sf);
+    rtile_sweep_k<C, true, FM, SKIP>(a, t, sp, sf, lo);
   else
-    rtile_sweep_k<C, false, FM>(a, t, sp, sf);
+    rtile_sweep_k<C, false, FM, SKIP>(a, t, sp, sf, lo);
 }
 
-template <bool FM, typename T, int RH, int NT>
+template <bool FM, bool SKIP, typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_passes_m(const SmoothArgs<T>& a,
                                                const RTile<T, RH, NT>& t, T* sp,
                                                const uint8_t* sf) {
   for (int pass = 0; pass < a.npass; ++pass) {
     if (((a.first_colour + pass) & 1) == 0)
-      rtile_sweep<0, FM>(a, t, sp, sf);
+      rtile_sweep<0, FM, SKIP>(a, t, sp, sf, pass + 1);
     else
-      rtile_sweep<1, FM>(a, t, sp, sf);
+      rtile_sweep<1, FM, SKIP>(a, t, sp, sf, pass + 1);
     __syncthreads();
   }
 }
 
-// npass colour passes over the whole region (the exact window shrinks by one
-// cell per pass).
+// Launches of more passes than this skip the row pairs outside each pass's
+// exact window (single-grid solves run up to 24 passes per launch).
+constexpr int RTILE_SKIP_PASSES = 8;
+
+// npass colour passes over the region (the exact window shrinks by one cell
+// per pass).
 template <typename T, int RH, int NT>
 __device__ __forceinline__ void rtile_passes(const SmoothArgs<T>& a, const RTile<T, RH, NT>& t,
                                              T* sp, const uint8_t* sf) {
-  if (sizeof(T) == 4 && a.fast_math)
-    rtile_passes_m<true>(a, t, sp, sf);
-  else
-    rtile_passes_m<false>(a, t, sp, sf);
+  const bool skip = a.npass > RTILE_SKIP_PASSES;
+  if (sizeof(T) == 4 && a.fast_math) {
+    if (skip)
+      rtile_passes_m<true, true>(a, t, sp, sf);
+    else
+      rtile_passes_m<true, false>(a, t, sp, sf);
+  } else {
+    if (skip)
+      rtile_passes_m<false, true>(a, t, sp, sf);
+    else
+      rtile_passes_m<false, false>(a, t, sp, sf);
+  }
 }
 
 // Write the tile interior of p; then the residuals of the lane's own cells
