@@ -166,10 +166,9 @@ struct smk_batch {
   int* d_count = nullptr;
   int* d_parity = nullptr;
   // pinned ring of active counts, one slot per queued iteration (4 slots),
-  // and its device address; ring_seq numbers the publications
+  // and its device address (publications numbered by next_ring_sequence)
   smk::CountSlot* h_ring = nullptr;
   smk::CountSlot* d_ring = nullptr;
-  unsigned ring_seq = 1;
   // V-cycle
   std::vector<smk::BatchDepth> depth;
   smk::SmoothArgs<float>* d_args = nullptr;
@@ -599,8 +598,7 @@ int smk_batch_step(smk_batch* b, smk_batch_info* info, void* stream) {
   // ring; the host polls the slot of iteration k-1 -- no copy, event or
   // wait on the stream per iteration.
   volatile CountSlot* ring = b->h_ring;
-  const unsigned base = b->ring_seq;
-  b->ring_seq = base + (unsigned)b->prm.max_cycles + 4u;
+  const unsigned base = next_ring_sequence((unsigned)b->prm.max_cycles + 4u);
   int it = 0, done_at = -1;
   for (; it <= b->prm.max_cycles + 1; ++it) {
     if (graphs) {

@@ -1995,15 +1995,20 @@ __global__ __launch_bounds__(NB_T) void k_residual_norm(const T* __restrict__ p,
   double acc = 0.0;
   if (i < W && j < H) {
     const size_t o = (size_t)j * W + i;
+    // as in k_correct_defect: every operand loaded (clamped indices) before
+    // the flags are tested
+    const size_t n = (size_t)H * W;
     const uint8_t f = flags[o];
+    const T ce = p[o], pl = p[o > 0 ? o - 1 : 0], pr = p[o + 1 < n ? o + 1 : o];
+    const T pd = p[o >= (size_t)W ? o - W : o], pu = p[o + W < n ? o + W : o];
+    const T b = rhs[o];
     double rv = 0.0;
     if (f & F_LFLUID) {
-      const T ce = p[o];
-      const T l = (f & F_LEFT) ? p[o - 1] : ce;
-      const T rr = (f & F_RIGHT) ? p[o + 1] : ce;
-      const T d = (f & F_DOWN) ? p[o - W] : ce;
-      const T u = (f & F_UP) ? p[o + W] : ce;
-      rv = poisson_residual(l, rr, d, u, ce, rhs[o], alpha);
+      const T l = (f & F_LEFT) ? pl : ce;
+      const T rr = (f & F_RIGHT) ? pr : ce;
+      const T d = (f & F_DOWN) ? pd : ce;
+      const T u = (f & F_UP) ? pu : ce;
+      rv = poisson_residual(l, rr, d, u, ce, b, alpha);
       acc = rv * rv;
     }
     if (defect != nullptr) defect[o] = (f & F_LFLUID) ? (float)(rv / alpha) : 0.0f;
@@ -2036,17 +2041,24 @@ __global__ __launch_bounds__(NB_T) void k_correct_defect(const T* __restrict__ p
   double acc = 0.0;
   if (i < W && j < H) {
     const size_t o = (size_t)j * W + i;
+    // the five corrected values are loaded at clamped indices before the
+    // flags are tested (a closed face's value is never used), so a lane's
+    // loads start together instead of waiting for its flags
+    const size_t n = (size_t)H * W;
+    const size_t ol = o > 0 ? o - 1 : 0, orr = o + 1 < n ? o + 1 : o;
+    const size_t od = o >= (size_t)W ? o - W : o, ou = o + W < n ? o + W : o;
     auto q = [&](size_t k) -> T { return (p_in != nullptr ? p_in[k] : (T)0) + (T)e[k]; };
     const uint8_t f = flags[o];
-    const T ce = q(o);
+    const T ce = q(o), ql = q(ol), qr = q(orr), qd = q(od), qu = q(ou);
+    const T b = rhs[o];
     p_out[o] = ce;
     double rv = 0.0;
     if (f & F_LFLUID) {
-      const T l = (f & F_LEFT) ? q(o - 1) : ce;
-      const T rr = (f & F_RIGHT) ? q(o + 1) : ce;
-      const T d = (f & F_DOWN) ? q(o - W) : ce;
-      const T u = (f & F_UP) ? q(o + W) : ce;
-      rv = poisson_residual(l, rr, d, u, ce, rhs[o], alpha);
+      const T l = (f & F_LEFT) ? ql : ce;
+      const T rr = (f & F_RIGHT) ? qr : ce;
+      const T d = (f & F_DOWN) ? qd : ce;
+      const T u = (f & F_UP) ? qu : ce;
+      rv = poisson_residual(l, rr, d, u, ce, b, alpha);
       acc = rv * rv;
     }
     defect[o] = (f & F_LFLUID) ? (float)(rv / alpha) : 0.0f;
@@ -2072,12 +2084,13 @@ __global__ __launch_bounds__(NB_T) void k_rhs_norm(const T* __restrict__ rhs,
   if (i < W && j < H) {
     const size_t o = (size_t)j * W + i;
     const bool fl = (flags[o] & F_LFLUID) != 0;
+    const T r = rhs[o];  // loaded with the flags, not after their test
     if (fl) {
-      const double b = alpha * (double)rhs[o];
+      const double b = alpha * (double)r;
       acc = b * b;
     }
     if (defect != nullptr) {
-      const double rv = fl ? poisson_residual((T)0, (T)0, (T)0, (T)0, (T)0, rhs[o], alpha) : 0.0;
+      const double rv = fl ? poisson_residual((T)0, (T)0, (T)0, (T)0, (T)0, r, alpha) : 0.0;
       defect[o] = fl ? (float)(rv / alpha) : 0.0f;
     }
   }
@@ -2750,12 +2763,22 @@ inline double optimal_omega(int size) {
 // the next one.  A right guess saves the no-op iteration for the host's
 // return latency; a wrong one costs the GPU that latency once.  The
 // iterations and their results are the same either way.
+// Sequence numbers of the pinned read-back rings (single-grid solves,
+// multigrid handles, batches), unique in the process.  A ring's memory can
+// be that of a freed ring, whose slots may still show the old bytes (a
+// publication that lands after the new ring was cleared); with every
+// publication numbered once per process, no such slot can pass for the
+// answer to a later check.
+inline unsigned next_ring_sequence(unsigned count) {
+  static std::atomic<unsigned> next{1u};
+  return next.fetch_add(count);
+}
+
 template <typename F>
-int run_gated_ring(SolveState* d_state, RingSlot* h_ring, RingSlot* d_ring, unsigned* seq,
-                   int max_iter, double tol, F&& iter, SolveState* fin, hipStream_t s) {
+int run_gated_ring(SolveState* d_state, RingSlot* h_ring, RingSlot* d_ring, int max_iter,
+                   double tol, F&& iter, SolveState* fin, hipStream_t s) {
   volatile RingSlot* ring = h_ring;
-  const unsigned base = *seq;
-  *seq = base + (unsigned)max_iter + 2u;
+  const unsigned base = next_ring_sequence((unsigned)max_iter + 2u);
   // while profiling, wait for each iteration before queueing the next, so
   // that no gated-off launch enters the per-kernel averages
   const int lag = g_prof_on ? 0 : 1;
@@ -2809,7 +2832,6 @@ int run_gated_ring(SolveState* d_state, RingSlot* h_ring, RingSlot* d_ring, unsi
 struct GateRing {
   RingSlot* h_ring = nullptr;
   RingSlot* d_ring = nullptr;  // device address of h_ring
-  unsigned seq = 1;
 };
 
 static int thread_gate_ring(GateRing** out) {
@@ -2837,7 +2859,7 @@ int sweeps_gated(SolverWs& w, int iterations, int check_every, double tol, F&& c
   rc = check_launch("k_state_init");
   if (rc) return rc;
   SolveState fin{};
-  rc = run_gated_ring(w.state, g->h_ring, g->d_ring, &g->seq, nchunks + 1, tol,
+  rc = run_gated_ring(w.state, g->h_ring, g->d_ring, nchunks + 1, tol,
                       [&](int k, RingSlot* slot, unsigned sq) -> int {
     int r = SMK_OK;
     if (k > 0) {
@@ -3065,7 +3087,6 @@ struct smk_mg {
   smk::SolveState* d_state;
   smk::RingSlot* h_ring;
   smk::RingSlot* d_ring;  // device address of h_ring
-  unsigned ring_seq;
   int small_from[2];  // first level handled by k_vcycle_small: [native dtype, float32]
   int small_mode[2];  // smoother mode small_from was computed for
   // float32 coarsest-level operator (SMK_COARSEST_OPERATOR) and what it was
@@ -3376,7 +3397,7 @@ int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, do
 // iteration queued after the last useful one is gated off and does nothing.
 template <typename F>
 int run_gated(smk_mg* mg, int max_iter, double tol, F&& iter, SolveState* fin, hipStream_t s) {
-  return run_gated_ring(mg->d_state, mg->h_ring, mg->d_ring, &mg->ring_seq, max_iter, tol, iter,
+  return run_gated_ring(mg->d_state, mg->h_ring, mg->d_ring, max_iter, tol, iter,
                         fin, s);
 }
 
@@ -4026,7 +4047,6 @@ int smk_mg_create(const uint8_t* fmu, const uint8_t* fmv, int H, int W, int num_
     hipError_t e = hipHostMalloc((void**)&mg->h_ring, 4 * sizeof(RingSlot), hipHostMallocDefault);
     if (e == hipSuccess) {
       memset(mg->h_ring, 0, 4 * sizeof(RingSlot));
-      mg->ring_seq = 1;
       e = hipHostGetDevicePointer((void**)&mg->d_ring, mg->h_ring, 0);
     }
     if (e != hipSuccess) rc = fail((int)e, "smk_mg_create: %s", hipGetErrorString(e));
