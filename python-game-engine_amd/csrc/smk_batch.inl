@@ -67,14 +67,18 @@ __global__ __launch_bounds__(NB_T) void k_b_defect(const BatchScene* __restrict_
       const size_t o = (size_t)j * W + i;
       const uint8_t f = d.flags0[o];
       const double* p = d.p;
+      // operands loaded (clamped indices) before the flags are tested
+      const size_t n = (size_t)H * W;
+      const double ce = p[o], pl = p[o > 0 ? o - 1 : 0], pr = p[o + 1 < n ? o + 1 : o];
+      const double pd = p[o >= (size_t)W ? o - W : o], pu = p[o + W < n ? o + W : o];
+      const double b = d.div[o];
       double rv = 0.0;
       if (f & F_LFLUID) {
-        const double ce = p[o];
-        const double l = (f & F_LEFT) ? p[o - 1] : ce;
-        const double rr = (f & F_RIGHT) ? p[o + 1] : ce;
-        const double dn = (f & F_DOWN) ? p[o - W] : ce;
-        const double up = (f & F_UP) ? p[o + W] : ce;
-        rv = poisson_residual(l, rr, dn, up, ce, d.div[o], d.alpha0);
+        const double l = (f & F_LEFT) ? pl : ce;
+        const double rr = (f & F_RIGHT) ? pr : ce;
+        const double dn = (f & F_DOWN) ? pd : ce;
+        const double up = (f & F_UP) ? pu : ce;
+        rv = poisson_residual(l, rr, dn, up, ce, b, d.alpha0);
         acc = rv * rv;
       }
       d.defect[o] = (f & F_LFLUID) ? (float)(rv / d.alpha0) : 0.0f;

@@ -222,13 +222,20 @@ __global__ void k_apply_forces(T* __restrict__ v, int vh, int vw,
   xcd_tile(bx_, by_);
   const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= vh || i >= vw) return;
+  // every operand loaded (rows clamped into the fields) before the edge
+  // cases select, so a lane's loads start together
+  const size_t o = (size_t)j * vw + i;
+  const T vo = v[o];
   float bf = 0.0f, tf = 0.0f;
+  const int ic = i < W ? i : W - 1;
+  const T r0 = rho[(size_t)(j - 1 < 0 ? 0 : (j - 1 < H ? j - 1 : H - 1)) * W + ic];
+  const T r1 = rho[(size_t)(j < H ? j : H - 1) * W + ic];
   if (j < H && i < W) {
     double d;
     if (j == 0 || j == H - 1) {
-      d = (double)rho[(size_t)j * W + i];
+      d = (double)r1;
     } else {
-      const T s = rho[(size_t)(j - 1) * W + i] + rho[(size_t)j * W + i];
+      const T s = r0 + r1;
       d = 0.5 * (double)s;
     }
     bf = (float)(-dc * d * g);
@@ -237,19 +244,22 @@ __global__ void k_apply_forces(T* __restrict__ v, int vh, int vw,
     double t;
     if (temp == nullptr) {
       t = 0.0;
-    } else if (j == 0) {
-      t = (double)temp[i];
-    } else if (j == Ht) {
-      t = (double)temp[(size_t)(Ht - 1) * Wt + i];
     } else {
-      const T s = temp[(size_t)(j - 1) * Wt + i] + temp[(size_t)j * Wt + i];
-      t = 0.5 * (double)s;
+      const T t0 = temp[(size_t)(j - 1 < 0 ? 0 : (j - 1 < Ht ? j - 1 : Ht - 1)) * Wt + i];
+      const T t1 = temp[(size_t)(j < Ht ? j : Ht - 1) * Wt + i];
+      if (j == 0) {
+        t = (double)t1;
+      } else if (j == Ht) {
+        t = (double)t0;
+      } else {
+        const T s = t0 + t1;
+        t = 0.5 * (double)s;
+      }
     }
     tf = (float)(-bc * (t - Ta));
   }
   const float s = bf + tf;
-  const size_t o = (size_t)j * vw + i;
-  v[o] = (T)((double)v[o] + dt * (double)s);
+  v[o] = (T)((double)vo + dt * (double)s);
 }
 
 template <typename T>
@@ -364,16 +374,23 @@ __global__ void k_residual_masks(const T* __restrict__ p, const T* __restrict__ 
   const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
   if (j >= H || i >= W) return;
   const size_t o = (size_t)j * W + i;
-  if (!fluid[o]) { r[o] = (T)0; return; }
   const size_t a = (size_t)j * (W + 1) + i;
+  // every operand loaded (clamped indices) before the masks are tested
+  const bool fl = fluid[o] != 0;
+  const uint8_t mu0 = fmu[a], mu1 = fmu[a + 1], mv0 = fmv[o], mv1 = fmv[o + W];
   const T c = p[o];
-  const T left = fmu[a] ? p[(size_t)j * W + (i - 1 > 0 ? i - 1 : 0)] : c;
-  const T right = fmu[a + 1] ? p[(size_t)j * W + (i + 1 < W - 1 ? i + 1 : W - 1)] : c;
-  const T down = fmv[o] ? p[(size_t)(j - 1 > 0 ? j - 1 : 0) * W + i] : c;
-  const T up = fmv[o + W] ? p[(size_t)(j + 1 < H - 1 ? j + 1 : H - 1) * W + i] : c;
+  const T pl = p[(size_t)j * W + (i - 1 > 0 ? i - 1 : 0)];
+  const T pr = p[(size_t)j * W + (i + 1 < W - 1 ? i + 1 : W - 1)];
+  const T pd = p[(size_t)(j - 1 > 0 ? j - 1 : 0) * W + i];
+  const T pu = p[(size_t)(j + 1 < H - 1 ? j + 1 : H - 1) * W + i];
+  const T dv = div[o];
+  const T left = mu0 ? pl : c;
+  const T right = mu1 ? pr : c;
+  const T down = mv0 ? pd : c;
+  const T up = mv1 ? pu : c;
   // neighbour sum in the array type, alpha/beta terms in float64 (Numba typing)
   const T nsum = left + right + down + up;
-  r[o] = (T)(((double)nsum + alpha * (double)div[o]) - beta * (double)c);
+  r[o] = fl ? (T)(((double)nsum + alpha * (double)dv) - beta * (double)c) : (T)0;
 }
 
 // The step's metrics pass (simulation.py:251-265): compute_divergence of the
@@ -399,17 +416,20 @@ __global__ void k_div_residual(const T* __restrict__ u, const T* __restrict__ v,
   const T ddv = (v[o + W] * (T)mv1) - (v[o] * (T)mv0);
   const T d = (T)((double)ddu * inv + (double)ddv * inv);
   div[o] = d;
-  if (!fluid[o]) {
-    r[o] = (T)0;
-    return;
-  }
+  // the pressures are loaded (clamped indices) with everything else, before
+  // the masks are tested, so a lane's loads start together
+  const bool fl = fluid[o] != 0;
   const T c = p[o];
-  const T left = mu0 ? p[(size_t)j * W + (i - 1 > 0 ? i - 1 : 0)] : c;
-  const T right = mu1 ? p[(size_t)j * W + (i + 1 < W - 1 ? i + 1 : W - 1)] : c;
-  const T down = mv0 ? p[(size_t)(j - 1 > 0 ? j - 1 : 0) * W + i] : c;
-  const T up = mv1 ? p[(size_t)(j + 1 < H - 1 ? j + 1 : H - 1) * W + i] : c;
+  const T pl = p[(size_t)j * W + (i - 1 > 0 ? i - 1 : 0)];
+  const T pr = p[(size_t)j * W + (i + 1 < W - 1 ? i + 1 : W - 1)];
+  const T pd = p[(size_t)(j - 1 > 0 ? j - 1 : 0) * W + i];
+  const T pu = p[(size_t)(j + 1 < H - 1 ? j + 1 : H - 1) * W + i];
+  const T left = mu0 ? pl : c;
+  const T right = mu1 ? pr : c;
+  const T down = mv0 ? pd : c;
+  const T up = mv1 ? pu : c;
   const T nsum = left + right + down + up;
-  r[o] = (T)(((double)nsum + alpha * (double)d) - beta * (double)c);
+  r[o] = fl ? (T)(((double)nsum + alpha * (double)d) - beta * (double)c) : (T)0;
 }
 
 // ------------------------------------------------------------- emitters
