@@ -1650,7 +1650,9 @@ __device__ __forceinline__ void small_smooth(T* p, T* tmp, const T* rhs, const u
 // registers: lane (bx, by) = (lane & 7, lane >> 3) owns cells (2bx+a, 2by+b).
 // A pass reads only values not written in that pass (the other colour, or the
 // previous Jacobi iterate), taken from the lane's own registers or from the
-// neighbouring lane with __shfl, so no pass touches LDS or a barrier.
+// neighbouring lane (DPP across columns, __shfl across rows), so no pass
+// touches LDS memory or a barrier.  A pass is one wave's chain of dependent
+// instructions, so everything constant across passes is taken out of it.
 // Neighbour values from lanes outside the level are never used: the cell
 // flags of border cells have those neighbour bits cleared.
 template <typename T, int MODE>
@@ -1658,51 +1660,63 @@ __device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t*
                               double omega, int npass) {
   const int lane = threadIdx.x & 63;
   const int bx = lane & 7, by = lane >> 3;
-  T c[2][2], r[2][2];
-  uint8_t f[2][2];
+  T c[2][2];
+  double ab[2][2];  // alpha * rhs, the same product every pass
+  // the cell's flags as lane masks, tested once instead of every pass
+  bool fluid[2][2], oL[2][2], oR[2][2], oD[2][2], oU[2][2];
 #pragma unroll
   for (int b = 0; b < 2; ++b)
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
       const int i = 2 * bx + a, j = 2 * by + b;
       const bool in = i < W && j < H;
+      const uint8_t f = in ? fl[j * W + i] : 0;
       c[b][a] = in ? p[j * W + i] : (T)0;
-      r[b][a] = in ? rhs[j * W + i] : (T)0;
-      f[b][a] = in ? fl[j * W + i] : 0;
+      ab[b][a] = alpha * (double)(in ? rhs[j * W + i] : (T)0);
+      fluid[b][a] = (f & F_FLUID) != 0;
+      oL[b][a] = (f & F_LEFT) != 0;
+      oR[b][a] = (f & F_RIGHT) != 0;
+      oD[b][a] = (f & F_DOWN) != 0;
+      oU[b][a] = (f & F_UP) != 0;
     }
-  const int lm = (lane - 1) & 63, lp = (lane + 1) & 63;
+  // vertical neighbours are 8 lanes away (a row of 2x2 blocks is 8 lanes):
+  // through the LDS crossbar; horizontal ones are the adjacent lanes: DPP
+  // (lane 0 / 63 keep their own value where __shfl would wrap; both are at
+  // the level's edge, whose faces are closed, so the value is never used)
   const int dm = (lane - 8) & 63, dp = (lane + 8) & 63;
+  // relax_value / poisson_new of the cell, in the same order: the neighbour
+  // sum in T, (sum + alpha * rhs) * 0.25 and the SOR blend in float64
   auto relax = [&](int a, int b, T left, T right, T down, T up) -> T {
-    const uint8_t ff = f[b][a];
     const T ce = c[b][a];
-    if (!(ff & F_FLUID)) return ce;
-    const T l = (ff & F_LEFT) ? left : ce;
-    const T rr = (ff & F_RIGHT) ? right : ce;
-    const T d = (ff & F_DOWN) ? down : ce;
-    const T u = (ff & F_UP) ? up : ce;
-    const double nw = poisson_new(l, rr, d, u, r[b][a], alpha);
-    return MODE == MODE_RB ? (T)((1.0 - omega) * (double)ce + omega * nw) : (T)nw;
+    const T l = oL[b][a] ? left : ce;
+    const T rr = oR[b][a] ? right : ce;
+    const T d = oD[b][a] ? down : ce;
+    const T u = oU[b][a] ? up : ce;
+    const T sum = l + rr + d + u;
+    const double nw = ((double)sum + ab[b][a]) * 0.25;
+    const T v = MODE == MODE_RB ? (T)((1.0 - omega) * (double)ce + omega * nw) : (T)nw;
+    return fluid[b][a] ? v : ce;
   };
   for (int pass = 0; pass < npass; ++pass) {
     if (MODE == MODE_RB) {
       if ((pass & 1) == 0) {  // colour 0: cells (0,0) and (1,1)
-        const T x01 = __shfl(c[0][1], lm), x10 = __shfl(c[1][0], dm);
-        const T y10 = __shfl(c[1][0], lp), y01 = __shfl(c[0][1], dp);
+        const T x01 = from_left(c[0][1]), x10 = __shfl(c[1][0], dm);
+        const T y10 = from_right(c[1][0]), y01 = __shfl(c[0][1], dp);
         const T n00 = relax(0, 0, x01, c[0][1], x10, c[1][0]);
         const T n11 = relax(1, 1, c[1][0], y10, c[0][1], y01);
         c[0][0] = n00;
         c[1][1] = n11;
       } else {  // colour 1: cells (1,0) and (0,1)
-        const T x00 = __shfl(c[0][0], lp), x11 = __shfl(c[1][1], dm);
-        const T y11 = __shfl(c[1][1], lm), y00 = __shfl(c[0][0], dp);
+        const T x00 = from_right(c[0][0]), x11 = __shfl(c[1][1], dm);
+        const T y11 = from_left(c[1][1]), y00 = __shfl(c[0][0], dp);
         const T n10 = relax(1, 0, c[0][0], x00, x11, c[1][1]);
         const T n01 = relax(0, 1, y11, c[1][1], c[0][0], y00);
         c[0][1] = n10;
         c[1][0] = n01;
       }
     } else {
-      const T l0 = __shfl(c[0][1], lm), l1 = __shfl(c[1][1], lm);
-      const T r0 = __shfl(c[0][0], lp), r1 = __shfl(c[1][0], lp);
+      const T l0 = from_left(c[0][1]), l1 = from_left(c[1][1]);
+      const T r0 = from_right(c[0][0]), r1 = from_right(c[1][0]);
       const T d0 = __shfl(c[1][0], dm), d1 = __shfl(c[1][1], dm);
       const T u0 = __shfl(c[0][0], dp), u1 = __shfl(c[0][1], dp);
       const T n00 = relax(0, 0, l0, c[0][1], d0, c[1][0]);
