@@ -1697,28 +1697,40 @@ __device__ __forceinline__ void coarsest_wave(T* p, const T* rhs, const uint8_t*
     const T v = MODE == MODE_RB ? (T)((1.0 - omega) * (double)ce + omega * nw) : (T)nw;
     return fluid[b][a] ? v : ce;
   };
-  for (int pass = 0; pass < npass; ++pass) {
-    if (MODE == MODE_RB) {
-      if ((pass & 1) == 0) {  // colour 0: cells (0,0) and (1,1)
-        const T x01 = from_left(c[0][1]), x10 = __shfl(c[1][0], dm);
-        const T y10 = from_right(c[1][0]), y01 = __shfl(c[0][1], dp);
-        const T n00 = relax(0, 0, x01, c[0][1], x10, c[1][0]);
-        const T n11 = relax(1, 1, c[1][0], y10, c[0][1], y01);
-        c[0][0] = n00;
-        c[1][1] = n11;
-      } else {  // colour 1: cells (1,0) and (0,1)
-        const T x00 = from_right(c[0][0]), x11 = __shfl(c[1][1], dm);
-        const T y11 = from_left(c[1][1]), y00 = __shfl(c[0][0], dp);
-        const T n10 = relax(1, 0, c[0][0], x00, x11, c[1][1]);
-        const T n01 = relax(0, 1, y11, c[1][1], c[0][0], y00);
-        c[0][1] = n10;
-        c[1][0] = n01;
-      }
-    } else {
-      const T l0 = from_left(c[0][1]), l1 = from_left(c[1][1]);
-      const T r0 = from_right(c[0][0]), r1 = from_right(c[1][0]);
+  // Each pass issues its LDS-crossbar reads before anything waits on one, so
+  // a pass costs one crossbar round trip.
+  auto colour0 = [&]() {  // cells (0,0) and (1,1)
+    const T x10 = __shfl(c[1][0], dm), y01 = __shfl(c[0][1], dp);
+    __builtin_amdgcn_sched_barrier(0);
+    const T x01 = from_left(c[0][1]), y10 = from_right(c[1][0]);
+    const T n00 = relax(0, 0, x01, c[0][1], x10, c[1][0]);
+    const T n11 = relax(1, 1, c[1][0], y10, c[0][1], y01);
+    c[0][0] = n00;
+    c[1][1] = n11;
+  };
+  auto colour1 = [&]() {  // cells (1,0) and (0,1)
+    const T x11 = __shfl(c[1][1], dm), y00 = __shfl(c[0][0], dp);
+    __builtin_amdgcn_sched_barrier(0);
+    const T x00 = from_right(c[0][0]), y11 = from_left(c[1][1]);
+    const T n10 = relax(1, 0, c[0][0], x00, x11, c[1][1]);
+    const T n01 = relax(0, 1, y11, c[1][1], c[0][0], y00);
+    c[0][1] = n10;
+    c[1][0] = n01;
+  };
+  if (MODE == MODE_RB) {
+    int pass = 0;
+    for (; pass + 2 <= npass; pass += 2) {
+      colour0();
+      colour1();
+    }
+    if (pass < npass) colour0();
+  } else {
+    for (int pass = 0; pass < npass; ++pass) {
       const T d0 = __shfl(c[1][0], dm), d1 = __shfl(c[1][1], dm);
       const T u0 = __shfl(c[0][0], dp), u1 = __shfl(c[0][1], dp);
+      __builtin_amdgcn_sched_barrier(0);
+      const T l0 = from_left(c[0][1]), l1 = from_left(c[1][1]);
+      const T r0 = from_right(c[0][0]), r1 = from_right(c[1][0]);
       const T n00 = relax(0, 0, l0, c[0][1], d0, c[1][0]);
       const T n10 = relax(1, 0, c[0][0], r0, d1, c[1][1]);
       const T n01 = relax(0, 1, l1, c[1][1], c[0][0], u0);
