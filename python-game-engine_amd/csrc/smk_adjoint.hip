@@ -237,8 +237,8 @@ __global__ void k_adj_scalar_sens(const T* __restrict__ ybar, const uint8_t* __r
     const size_t o = (size_t)j * W + i;
     const double y = fluid[o] ? (double)ybar[o] : 0.0;
     const double px = i + 0.5, py = j + 0.5;
-    const double uu = sample_field(u, H, W + 1, px, py - 0.5);
-    const double vv = sample_field(v, H + 1, W, px - 0.5, py);
+    const double uu = sample_known(u, H, W + 1, px, py - 0.5, i, i + 1, j, j);
+    const double vv = sample_known(v, H + 1, W, px - 0.5, py, i, i, j, j + 1);
     const double x = px - (uu * dt * inv);
     const double yy = py - (vv * dt * inv);
     const double sx = x - 0.5, sy = yy - 0.5;
@@ -305,7 +305,7 @@ __global__ void k_adj_vel_sens(const T* __restrict__ udbar, const T* __restrict_
     const double y = fmu[o] ? (double)udbar[o] : 0.0;
     const double px = (double)i, py = j + 0.5;
     const double uu = (double)u[o];
-    const double vv = sample_field(v, H + 1, W, px - 0.5, py);
+    const double vv = sample_known(v, H + 1, W, px - 0.5, py, i - 1, i, j, j + 1);
     const double x = px - (uu * dt * inv);
     const double yy = py - (vv * dt * inv);
     const double sx = x, sy = yy - 0.5;
@@ -323,7 +323,7 @@ __global__ void k_adj_vel_sens(const T* __restrict__ udbar, const T* __restrict_
     const size_t o = (size_t)j * W + i;
     const double y = fmv[o] ? (double)vdbar[o] : 0.0;
     const double px = i + 0.5, py = (double)j;
-    const double uu = sample_field(u, H, W + 1, px, py - 0.5);
+    const double uu = sample_known(u, H, W + 1, px, py - 0.5, i, i + 1, j - 1, j);
     const double vv = (double)v[o];
     const double x = px - (uu * dt * inv);
     const double yy = py - (vv * dt * inv);

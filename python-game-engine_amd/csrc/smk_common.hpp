@@ -62,22 +62,120 @@ __device__ __forceinline__ int clamp_index(double f, int hi) {
   return (int)f;
 }
 
+// clamp_index(ceil(x), hi) from x0 = clamp_index(floor(x), hi).  The floor
+// itself is not needed: x <= 0 and NaN clamp to 0 either way, and for x > 0
+// the truncation in clamp_index is the floor.  Above x0 the ceiling is
+// x0 + 1 unless x is a whole number; both forms agree for every double
+// (including the infinities and NaN) and clamp bound.
+__device__ __forceinline__ int clamp_ceil(double x, int x0, int hi) {
+  const int c = x0 + ((double)x0 < x ? 1 : 0);
+  return c < hi ? c : hi;
+}
+
+// clamp_index of a whole number held as an int
+__device__ __forceinline__ int clamp_whole(int v, int hi) { return v > 0 ? (v < hi ? v : hi) : 0; }
+
+// The lerps of sample_field from the four corner values (cYX: row Y,
+// column X) and the weights tx = x - x0, ty = y - y0.
 template <typename T>
-__device__ __forceinline__ double sample_field(const T* __restrict__ f, int fh,
-                                               int fw, double x, double y) {
-  const int x0 = clamp_index(floor(x), fw - 1);
-  const int x1 = clamp_index(ceil(x), fw - 1);
-  const int y0 = clamp_index(floor(y), fh - 1);
-  const int y1 = clamp_index(ceil(y), fh - 1);
-  const double tx = x - (double)x0;
-  const double ty = y - (double)y0;
+__device__ __forceinline__ double lerp4(T c00, T c10, T c01, T c11, double tx, double ty) {
+  const double c0 = (double)c00 + (double)(T)(c10 - c00) * tx;
+  const double c1 = (double)c01 + (double)(T)(c11 - c01) * tx;
+  return c0 + (c1 - c0) * ty;
+}
+
+// The lerps of sample_field once the four clamped indices are known.
+template <typename T>
+__device__ __forceinline__ double bilerp(const T* __restrict__ f, int fw, int x0, int x1, int y0,
+                                         int y1, double x, double y) {
   const T c00 = f[(size_t)y0 * fw + x0];
   const T c10 = f[(size_t)y0 * fw + x1];
   const T c01 = f[(size_t)y1 * fw + x0];
   const T c11 = f[(size_t)y1 * fw + x1];
-  const double c0 = (double)c00 + (double)(T)(c10 - c00) * tx;
-  const double c1 = (double)c01 + (double)(T)(c11 - c01) * tx;
-  return c0 + (c1 - c0) * ty;
+  return lerp4(c00, c10, c01, c11, x - (double)x0, y - (double)y0);
+}
+
+template <typename T>
+__device__ __forceinline__ double sample_field(const T* __restrict__ f, int fh,
+                                               int fw, double x, double y) {
+  const int x0 = clamp_index(x, fw - 1);
+  const int y0 = clamp_index(y, fh - 1);
+  return bilerp(f, fw, x0, clamp_ceil(x, x0, fw - 1), y0, clamp_ceil(y, y0, fh - 1), x, y);
+}
+
+// Two consecutive elements of a row, read as one load (the hardware takes
+// 8- and 16-byte global loads at the element's own alignment).
+template <typename T>
+struct __attribute__((aligned(sizeof(T)))) Pair {
+  T a, b;
+};
+template <typename T>
+__device__ __forceinline__ Pair<T> load_pair(const T* __restrict__ p) {
+  Pair<T> r;
+  __builtin_memcpy(&r, p, sizeof(r));
+  return r;
+}
+
+// sample_field for a field in global memory, each row's two corners read by
+// one load.  x1 is x0 or x0 + 1, so the pair starting at
+// xs = min(x0, fw - 2) holds both; the corner values, weights and lerps are
+// sample_field's.  A one-column field reads its corners singly.
+template <typename T>
+__device__ __forceinline__ double sample_field_rows(const T* __restrict__ f, int fh, int fw,
+                                                    double x, double y) {
+  const int x0 = clamp_index(x, fw - 1);
+  const int y0 = clamp_index(y, fh - 1);
+  const int x1 = clamp_ceil(x, x0, fw - 1), y1 = clamp_ceil(y, y0, fh - 1);
+  if (fw < 2) return bilerp(f, fw, x0, x1, y0, y1, x, y);
+  const int xs = x0 < fw - 2 ? x0 : fw - 2;
+  const Pair<T> r0 = load_pair(f + (size_t)y0 * fw + xs);
+  const Pair<T> r1 = load_pair(f + (size_t)y1 * fw + xs);
+  const bool a0 = x0 == xs, a1 = x1 == xs;
+  return lerp4(a0 ? r0.a : r0.b, a1 ? r0.a : r0.b, a0 ? r1.a : r1.b, a1 ? r1.a : r1.b,
+               x - (double)x0, y - (double)y0);
+}
+
+// sample_field at a point whose floor and ceiling are known whole numbers
+// (fx, cx, fy, cy): the staggered-grid averages of the advection step, whose
+// positions are a cell centre or face plus a multiple of one half.  Same
+// indices, weights and lerps as sample_field, without the float64 rounding
+// and comparisons that find them.
+template <typename T>
+__device__ __forceinline__ double sample_known(const T* __restrict__ f, int fh, int fw, double x,
+                                               double y, int fx, int cx, int fy, int cy) {
+  return bilerp(f, fw, clamp_whole(fx, fw - 1), clamp_whole(cx, fw - 1), clamp_whole(fy, fh - 1),
+                clamp_whole(cy, fh - 1), x, y);
+}
+
+// --------------------------------------------------------- lane moves
+// Apply a 32-bit lane operation to every dword of a value.
+template <typename T, typename F>
+__device__ __forceinline__ T per_dword(T v, F f) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized values only");
+  uint32_t w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = f(w[i]);
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+// Lane l receives lane l-1's value, lane 0 keeps its own (DPP wave_shr:1;
+// the result of __shfl_up(v, 1, 64) without a trip through the LDS
+// crossbar).  All 64 lanes must be active.
+template <typename T>
+__device__ __forceinline__ T from_left(T v) {
+  return per_dword(v, [](uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x138, 0xf, 0xf, false);
+  });
+}
+// Lane l receives lane l+1's value, lane 63 keeps its own (DPP wave_shl:1).
+template <typename T>
+__device__ __forceinline__ T from_right(T v) {
+  return per_dword(v, [](uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x130, 0xf, 0xf, false);
+  });
 }
 
 // --------------------------------------------------------- XCD-aware order

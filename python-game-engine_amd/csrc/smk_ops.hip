@@ -133,11 +133,12 @@ __global__ void k_advect_scalar(const T* __restrict__ src, T* __restrict__ dst, 
   // selects, so the loads of a lane start together
   const bool fl = fluid[o] != 0;
   const double px = i + 0.5, py = j + 0.5;
-  const double uu = sample_field(u, uh, uw, px, py - 0.5);
-  const double vv = sample_field(v, vh, vw, px - 0.5, py);
+  // u at (i + 1/2, j) and v at (i, j + 1/2): floors and ceilings known
+  const double uu = sample_known(u, uh, uw, px, py - 0.5, i, i + 1, j, j);
+  const double vv = sample_known(v, vh, vw, px - 0.5, py, i, i, j, j + 1);
   const double x = px - (uu * dt * inv);
   const double y = py - (vv * dt * inv);
-  T r = (T)sample_field(src, H, W, x - 0.5, y - 0.5);
+  T r = (T)sample_field_rows(src, H, W, x - 0.5, y - 0.5);
   if (apply_scale) r = r * (T)scale;
   dst[o] = fl ? r : (T)0;
 }
@@ -152,7 +153,7 @@ __device__ __forceinline__ void advect_u_at(const T* __restrict__ us, T* __restr
   const bool open = fmu[o] != 0;
   const double px = (double)i, py = j + 0.5;
   const double uu = (double)us[o];
-  const double vv = sample_field(v, vh, vw, px - 0.5, py);
+  const double vv = sample_known(v, vh, vw, px - 0.5, py, i - 1, i, j, j + 1);
   const double x = px - (uu * dt * inv);
   const double y = py - (vv * dt * inv);
   const T r = (T)sample_field(us, uh, uw, x, y - 0.5);
@@ -167,12 +168,44 @@ __device__ __forceinline__ void advect_v_at(const T* __restrict__ vs, T* __restr
   const size_t o = (size_t)j * vw + i;
   const bool open = fmv[o] != 0;
   const double px = i + 0.5, py = (double)j;
-  const double uu = sample_field(u, uh, uw, px, py - 0.5);
+  const double uu = sample_known(u, uh, uw, px, py - 0.5, i, i + 1, j - 1, j);
   const double vv = (double)vs[o];
   const double x = px - (uu * dt * inv);
   const double y = py - (vv * dt * inv);
   const T r = (T)sample_field(vs, vh, vw, x - 0.5, y);
   vd[o] = open ? r : (T)0;
+}
+
+// The u face (j, i) and the v face (j, i) of one thread in straight-line
+// code: both faces' first loads are issued before either face computes and
+// both traced samples before either store, so a thread waits on memory
+// twice, not four times.  The caller guarantees j <= H and i <= W; a thread
+// on the top row (no u face) or right column (no v face) traces a clamped
+// face and does not store it.  Per face, the same operations as advect_u_at
+// and advect_v_at.
+template <typename T>
+__device__ __forceinline__ void advect_uv_at(const T* __restrict__ us, T* __restrict__ ud,
+                                             const T* __restrict__ vs, T* __restrict__ vd,
+                                             const uint8_t* __restrict__ fmu,
+                                             const uint8_t* __restrict__ fmv, int H, int W,
+                                             double dt, double inv, int j, int i) {
+  const bool su = j < H, sv = i < W;
+  const int ju = su ? j : H - 1, iv = sv ? i : W - 1;
+  const int uw = W + 1, vh = H + 1;
+  const size_t ou = (size_t)ju * uw + i, ov = (size_t)j * W + iv;
+  const bool open_u = fmu[ou] != 0, open_v = fmv[ov] != 0;
+  const double pxu = (double)i, pyu = ju + 0.5;  // u face
+  const double pxv = iv + 0.5, pyv = (double)j;  // v face
+  const double uu_u = (double)us[ou];
+  const double vv_u = sample_known(vs, vh, W, pxu - 0.5, pyu, i - 1, i, ju, ju + 1);
+  const double uu_v = sample_known(us, H, uw, pxv, pyv - 0.5, iv, iv + 1, j - 1, j);
+  const double vv_v = (double)vs[ov];
+  const double xu = pxu - (uu_u * dt * inv), yu = pyu - (vv_u * dt * inv);
+  const double xv = pxv - (uu_v * dt * inv), yv = pyv - (vv_v * dt * inv);
+  const T ru = (T)sample_field(us, H, uw, xu, yu - 0.5);
+  const T rv = (T)sample_field(vs, vh, W, xv - 0.5, yv);
+  if (su) ud[ou] = open_u ? ru : (T)0;
+  if (sv) vd[ov] = open_v ? rv : (T)0;
 }
 
 template <typename T>
@@ -196,17 +229,69 @@ __global__ void k_advect_v(const T* __restrict__ vs, T* __restrict__ vd, int vh,
 }
 
 // u and v faces of an H x W MAC grid in one launch over (H+1) x (W+1).
+// A wavefront is one row segment of 64 consecutive columns, so the
+// staggered averages take the neighbouring column from the adjacent lane
+// (DPP): a thread loads u at rows jr and jm and v at rows j and jr + 1 of
+// its own column, and lanes 0 and 63 load the one column beyond the
+// wavefront themselves.  Per face these are the values, weights and lerps
+// of advect_uv_at (the corner indices below are sample_known's clamped
+// ones wherever the face is stored).  Every lane of the padded grid stays
+// active, as DPP needs: indices are clamped into the grid and only faces
+// inside it are stored.
 template <typename T>
-__global__ void k_advect_velocity(const T* __restrict__ us, T* __restrict__ ud,
-                                  const T* __restrict__ vs, T* __restrict__ vd,
-                                  const uint8_t* __restrict__ fmu,
-                                  const uint8_t* __restrict__ fmv, int H, int W, double dt,
-                                  double inv) {
+__global__ __launch_bounds__(BX * BY) void k_advect_velocity(
+    const T* __restrict__ us, T* __restrict__ ud, const T* __restrict__ vs, T* __restrict__ vd,
+    const uint8_t* __restrict__ fmu, const uint8_t* __restrict__ fmv, int H, int W, double dt,
+    double inv) {
+  static_assert(BX == 64, "a wavefront is one 64-column row segment");
   int bx_, by_;
   xcd_tile(bx_, by_);
-  const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
-  if (j < H && i <= W) advect_u_at(us, ud, H, W + 1, vs, H + 1, W, fmu, dt, inv, j, i);
-  if (j <= H && i < W) advect_v_at(vs, vd, H + 1, W, us, H, W + 1, fmv, dt, inv, j, i);
+  const int lane = threadIdx.x;
+  const int gi = bx_ * BX + lane, gj = by_ * BY + threadIdx.y;
+  const bool su = gj < H && gi <= W;       // u face (gj, gi) exists
+  const bool sv = gj <= H && gi < W;       // v face (gj, gi) exists
+  const int i = gi < W ? gi : W;           // u column, in [0, W]
+  const int j = gj < H ? gj : H;           // v row, in [0, H]
+  const int jr = j < H ? j : H - 1;        // u row of the u face
+  const int jm = j > 0 ? j - 1 : 0;        // lower u row of the v face's average
+  const int ic = i < W ? i : W - 1;        // v column of the v face
+  const int uw = W + 1;
+  const size_t ou = (size_t)jr * uw + i, ov = (size_t)j * W + ic;
+  const T A = us[ou];                           // u(jr, i)
+  const T D = us[(size_t)jm * uw + i];          // u(jm, i)
+  const T B = vs[ov];                           // v(j, ic) = v(jr, ic) where su
+  const T C = vs[(size_t)(jr + 1) * W + ic];    // v(jr + 1, ic)
+  const bool open_u = fmu[ou] != 0, open_v = fmv[ov] != 0;
+  // v at column i - 1 (clamped) for the u face, u at column i + 1 for the v
+  // face: loaded by the wavefront's edge lanes (issued with the loads above,
+  // before anything waits), from the neighbouring lane elsewhere
+  T Be = (T)0, Ce = (T)0, Ae = (T)0, De = (T)0;
+  const bool left_load = lane == 0 && i > 0, right_load = lane == 63 && i < W;
+  if (left_load) {
+    Be = vs[(size_t)jr * W + i - 1];
+    Ce = vs[(size_t)(jr + 1) * W + i - 1];
+  }
+  if (right_load) {
+    Ae = us[(size_t)jr * uw + i + 1];
+    De = us[(size_t)jm * uw + i + 1];
+  }
+  const T Bn = from_left(B), Cn = from_left(C), An = from_right(A), Dn = from_right(D);
+  const T Bl = left_load ? Be : (lane == 0 ? B : Bn);
+  const T Cl = left_load ? Ce : (lane == 0 ? C : Cn);
+  const T Ar = right_load ? Ae : An;
+  const T Dr = right_load ? De : Dn;
+  const double pxu = (double)i, pyu = jr + 0.5;  // u face
+  const double pxv = ic + 0.5, pyv = (double)j;  // v face
+  const int x0u = i > 0 ? i - 1 : 0;
+  const double vv_u = lerp4(Bl, B, Cl, C, (pxu - 0.5) - (double)x0u, pyu - (double)jr);
+  const double uu_v = lerp4(D, Dr, A, Ar, pxv - (double)ic, (pyv - 0.5) - (double)jm);
+  const double uu_u = (double)A, vv_v = (double)B;
+  const double xu = pxu - (uu_u * dt * inv), yu = pyu - (vv_u * dt * inv);
+  const double xv = pxv - (uu_v * dt * inv), yv = pyv - (vv_v * dt * inv);
+  const T ru = (T)sample_field_rows(us, H, uw, xu, yu - 0.5);
+  const T rv = (T)sample_field_rows(vs, H + 1, W, xv - 0.5, yv);
+  if (su) ud[ou] = open_u ? ru : (T)0;
+  if (sv) vd[ov] = open_v ? rv : (T)0;
 }
 
 // --------------------------------------------------------------- forces
@@ -538,8 +623,8 @@ __global__ __launch_bounds__(BT_NT) void k_b_advect_velocity(const BatchScene* _
     const double inv = 1.0 / d.h;
     int i, j;
     job_pos(jb, i, j);
-    if (j < H && i <= W) advect_u_at(d.u, d.u_tmp, H, W + 1, d.v, H + 1, W, d.fmu, ph.dt, inv, j, i);
-    if (j <= H && i < W) advect_v_at(d.v, d.v_tmp, H + 1, W, d.u, H, W + 1, d.fmv, ph.dt, inv, j, i);
+    if (j <= H && i <= W)
+      advect_uv_at(d.u, d.u_tmp, d.v, d.v_tmp, d.fmu, d.fmv, H, W, ph.dt, inv, j, i);
   }
 }
 
@@ -565,8 +650,8 @@ __global__ __launch_bounds__(BT_NT) void k_b_advect_density(const BatchScene* __
     }
     const double inv = 1.0 / d.h;
     const double px = i + 0.5, py = j + 0.5;
-    const double uu = sample_field(d.u_tmp, H, W + 1, px, py - 0.5);
-    const double vv = sample_field(d.v_tmp, H + 1, W, px - 0.5, py);
+    const double uu = sample_known(d.u_tmp, H, W + 1, px, py - 0.5, i, i + 1, j, j);
+    const double vv = sample_known(d.v_tmp, H + 1, W, px - 0.5, py, i, i, j, j + 1);
     const double x = px - (uu * ph.dt * inv);
     const double y = py - (vv * ph.dt * inv);
     dst[o] = sample_field(src, H, W, x - 0.5, y - 0.5);

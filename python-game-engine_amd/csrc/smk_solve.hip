@@ -1023,36 +1023,6 @@ __global__ __launch_bounds__(NT, (d_waves<T, RH, NT>())) void k_smooth_tile64_d(
 // LDS holds only those edges, so occupancy is set by registers.  With R even
 // the 2x2 restriction blocks coincide with (lane pair, row pair).
 
-// Apply a 32-bit lane operation to every dword of a value.
-template <typename T, typename F>
-__device__ __forceinline__ T per_dword(T v, F f) {
-  static_assert(sizeof(T) % 4 == 0, "dword-sized values only");
-  uint32_t w[sizeof(T) / 4];
-  __builtin_memcpy(w, &v, sizeof(T));
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = f(w[i]);
-  T r;
-  __builtin_memcpy(&r, w, sizeof(T));
-  return r;
-}
-
-// Lane l receives lane l-1's value, lane 0 keeps its own (DPP wave_shr:1;
-// the result of __shfl_up(v, 1, 64) without a trip through the LDS
-// crossbar).  All 64 lanes must be active.
-template <typename T>
-__device__ __forceinline__ T from_left(T v) {
-  return per_dword(v, [](uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x138, 0xf, 0xf, false);
-  });
-}
-// Lane l receives lane l+1's value, lane 63 keeps its own (DPP wave_shl:1).
-template <typename T>
-__device__ __forceinline__ T from_right(T v) {
-  return per_dword(v, [](uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x130, 0xf, 0xf, false);
-  });
-}
-
 // Strip exchange for a wavefront split into 64 / COLS strips of COLS lanes:
 // lane l receives v from lane l - COLS if `down`, else from lane l + COLS
 // (lanes whose source lies outside the wave receive an unspecified value).
