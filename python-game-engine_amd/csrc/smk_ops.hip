@@ -391,42 +391,38 @@ __global__ void k_pressure_gradient(T* __restrict__ u, T* __restrict__ v,
   int bx_, by_;
   xcd_tile(bx_, by_);
   const int i = bx_ * BX + threadIdx.x, j = by_ * BY + threadIdx.y;
-  // every operand is loaded before any is tested (indices clamped into the
-  // arrays), so a lane's loads start together; masks and walls select
-  if (j < H && i <= W) {
-    const size_t o = (size_t)j * (W + 1) + i;
-    const int ir = i < W ? i : W - 1;          // open face past the end: clamped
-    const int il = i - 1 < 0 ? W - 1 : i - 1;  // index -1 wraps (Python)
-    const bool open = fmu[o] != 0;
-    const T uo = u[o];
-    const T dp = p[(size_t)j * W + ir] - p[(size_t)j * W + il];
-    T val = open ? (T)((double)uo - scale * (double)dp) : (T)0;
-    if (solid != nullptr) {
-      const uint8_t sl = solid[(size_t)j * W + (i > 0 ? i - 1 : 0)];
-      const uint8_t sr = solid[(size_t)j * W + (i < W ? i : W - 1)];
-      const bool ls = i > 0 && sl == 1;
-      const bool rs = i < W && sr == 1;
-      if (ls || rs) val = (T)0;
-    }
-    u[o] = val;
+  // Both faces of the thread in straight-line code: every operand of the u
+  // face (j, i) and the v face (j, i) is loaded, at indices clamped into the
+  // arrays, before any is tested, so a lane waits on memory once; masks and
+  // walls select, and a face outside the grid (u on the top row, v on the
+  // right column) is computed on clamped indices and not stored.  The two
+  // faces share the pressure p(j, i) and the wall flag solid(j, i).
+  if (j > H || i > W) return;
+  const int jc = j < H ? j : H - 1;          // row of the u face
+  const int ic = i < W ? i : W - 1;          // column of the v face
+  const int il = i - 1 < 0 ? W - 1 : i - 1;  // index -1 wraps (Python)
+  const int jb = j - 1 < 0 ? H - 1 : j - 1;
+  const size_t ou = (size_t)jc * (W + 1) + i, ov = (size_t)j * W + ic;
+  const bool open_u = fmu[ou] != 0, open_v = fmv[ov] != 0;
+  const T uo = u[ou], vo = v[ov];
+  const T pc = p[(size_t)jc * W + ic];       // p(j, ir) of the u face = p(jt, i) of the v face
+  const T pl = p[(size_t)jc * W + il];
+  const T pb = p[(size_t)jb * W + ic];
+  bool wall_u = false, wall_v = false;
+  if (solid != nullptr) {
+    const uint8_t sc = solid[(size_t)jc * W + ic];
+    const uint8_t sl = solid[(size_t)jc * W + (i > 0 ? i - 1 : 0)];
+    const uint8_t sb = solid[(size_t)(j > 0 ? j - 1 : 0) * W + ic];
+    wall_u = (i > 0 && sl == 1) || (i < W && sc == 1);
+    wall_v = (j > 0 && sb == 1) || (j < H && sc == 1);
   }
-  if (j <= H && i < W) {
-    const size_t o = (size_t)j * W + i;
-    const int jt = j < H ? j : H - 1;
-    const int jb = j - 1 < 0 ? H - 1 : j - 1;
-    const bool open = fmv[o] != 0;
-    const T vo = v[o];
-    const T dp = p[(size_t)jt * W + i] - p[(size_t)jb * W + i];
-    T val = open ? (T)((double)vo - scale * (double)dp) : (T)0;
-    if (solid != nullptr) {
-      const uint8_t st = solid[(size_t)(j > 0 ? j - 1 : 0) * W + i];
-      const uint8_t sb = solid[(size_t)(j < H ? j : H - 1) * W + i];
-      const bool ts = j > 0 && st == 1;
-      const bool bs = j < H && sb == 1;
-      if (ts || bs) val = (T)0;
-    }
-    v[o] = val;
-  }
+  const T dpu = pc - pl, dpv = pc - pb;
+  T val_u = open_u ? (T)((double)uo - scale * (double)dpu) : (T)0;
+  T val_v = open_v ? (T)((double)vo - scale * (double)dpv) : (T)0;
+  if (wall_u) val_u = (T)0;
+  if (wall_v) val_v = (T)0;
+  if (j < H) u[ou] = val_u;
+  if (i < W) v[ov] = val_v;
 }
 
 template <typename T>
