@@ -58,14 +58,24 @@ def log(msg: str) -> None:
 
 
 def algorithmic_bytes_per_cell(v_cycles_per_step: float, mixed: bool, levels: int,
-                               refinements: float = 0.0, coarsest_op: bool = False) -> float:
+                               refinements: float = 0.0, coarsest_op: bool = False,
+                               exact_level: int = 0, exact_cells: int = 0) -> float:
     """Bytes per cell-step by logical pass (float64 8 B, float32 4 B):
     non-solve passes 168, metrics divergence + residual 48, per FP64 V-cycle
     176.06 (V(2,2) smoothing 4x24, residual+restrict 18, prolong+correct 18,
     summed over levels x 4/3, plus 50 coarsest sweeps) + 16 for the norm
     check; mixed: per FP32 V-cycle 88.03, per refinement 40.  With
     ``coarsest_op`` the coarsest level is one read of its float32 256x256
-    operator (4 x 256 B per coarsest cell) in place of the sweeps."""
+    operator (4 x 256 B per coarsest cell) in place of the sweeps.  With
+    ``exact_cells`` > 0 (mixed, coarsest_solve "exact_1024") the FP32
+    V-cycles smooth levels 0 .. exact_level - 1 and read the float32 exact
+    operator of level exact_level, exact_cells^2 x 4 B, in place of that
+    level and those below."""
+    if mixed and exact_cells > 0:
+        geo = sum(0.25 ** l for l in range(exact_level))
+        op_per_cell = 4.0 * exact_cells * 0.25 ** exact_level  # 4 n^2 B over n / 0.25^l cells
+        return (168.0 + 48.0 + v_cycles_per_step * (66.0 * geo + op_per_cell)
+                + refinements * 40.0 + 16.0)
     geo = sum(0.25 ** l for l in range(levels - 1))
     coarse_cells = 0.25 ** (levels - 1)
     coarsest = 4.0 * 256 * coarse_cells if coarsest_op else 50 * 24.0 * coarse_cells
@@ -74,6 +84,16 @@ def algorithmic_bytes_per_cell(v_cycles_per_step: float, mixed: bool, levels: in
         per_vc = 66.0 * geo + (coarsest if coarsest_op else coarsest / 2.0)
         return 168.0 + 48.0 + v_cycles_per_step * per_vc + refinements * 40.0 + 16.0
     return 168.0 + 48.0 + v_cycles_per_step * (per_vc64 + 16.0) + 16.0
+
+
+def exact_level_of(N: int, max_cells: int = 1024) -> tuple:
+    """(level, cells) of the first level of an N x N hierarchy with at most
+    max_cells cells (sizes halve while even), as the solver picks it."""
+    n, level = N, 0
+    while n * n > max_cells and n % 2 == 0:
+        n //= 2
+        level += 1
+    return (level, n * n) if n * n <= max_cells else (0, 0)
 
 
 # finest-level launches repeated inside their event pair while profiling
@@ -587,11 +607,12 @@ def main():
     ap.add_argument("--variants", default="fp64,mixed")
     ap.add_argument("--mixed-inner", type=int, default=2,
                     help="FP32 V-cycles per FP64 refinement in the mixed variant")
-    ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator", "exact"],
-                    default="exact",
+    ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator", "exact", "exact_1024"],
+                    default="exact_1024",
                     help="coarsest level of the mixed variant's FP32 V-cycles: the 50 sweeps, "
-                         "those sweeps applied as their precomputed 256x256 operator, or the "
-                         "level's exact least-squares operator")
+                         "those sweeps applied as their precomputed 256x256 operator, the "
+                         "level's exact least-squares operator, or the exact operator of the "
+                         "32x32 level (the cycles stop there)")
     ap.add_argument("--mixed-arith", choices=["float64", "float32"], default="float32",
                     help="arithmetic of the mixed variant's FP32 V-cycle passes: the reference's "
                          "float64 expression, or float32")
@@ -693,9 +714,12 @@ def main():
         r["inner_cycles_per_refinement"] = inner
         r["coarsest_solve"] = coarsest
         r["inner_arithmetic"] = arith if var == "mixed" else None
+        ex_level, ex_cells = (exact_level_of(args.N) if var == "mixed" and
+                              coarsest == "exact_1024" else (0, 0))
         r["alg_bytes_per_cell_step"] = algorithmic_bytes_per_cell(
             r["v_cycles_per_step"], var == "mixed", r["levels"] or 8, r["refinements_per_step"],
-            coarsest_op=var == "mixed" and coarsest in ("operator", "exact"))
+            coarsest_op=var == "mixed" and coarsest in ("operator", "exact"),
+            exact_level=ex_level, exact_cells=ex_cells)
         r["alg_GBps_whole_step"] = r["alg_bytes_per_cell_step"] * r["cell_steps_per_s"] / 1e9
         results[var] = r
         log(f"variant {var}: {json.dumps(r)}")
@@ -753,9 +777,11 @@ def main():
         "config": {"workload": f"plume_{args.N}_mg", "grid": [args.N, args.N],
                    "solver": "multigrid V(2,2) RB-GS, 8 levels, coarsest 50 sweeps, tol 1e-8, "
                              "rediscretized coarse scale" + (
-                                 "; mixed: FP32 V-cycles, 16x16 level solved exactly"
-                                 if main_var == "mixed" and
-                                 results[main_var].get("coarsest_solve") == "exact" else ""),
+                                 {"exact": "; mixed: FP32 V-cycles, 16x16 level solved exactly",
+                                  "exact_1024": "; mixed: FP32 V-cycles to the 32x32 level, "
+                                                "solved exactly"}.get(
+                                     results[main_var].get("coarsest_solve"), "")
+                                 if main_var == "mixed" else ""),
                    "variant": main_var,
                    "coarsest_solve": results[main_var].get("coarsest_solve", "sweeps"),
                    "inner_arithmetic": results[main_var].get("inner_arithmetic")},

@@ -256,6 +256,14 @@ typedef struct smk_mg smk_mg;
                                    null space), built once on the host;
                                    coarsest_iterations and coarsest_omega do
                                    not enter (new, mixed precision only)     */
+#define SMK_COARSEST_EXACT_1024 3 /* as SMK_COARSEST_EXACT, for the first
+                                   level of at most 1024 cells (32x32 on a
+                                   power-of-two grid): the FP32 V-cycles stop
+                                   at that level and apply its exact
+                                   operator, a 1024x1024 float32 matrix, with
+                                   one multi-workgroup product; the levels
+                                   below it are not visited (new, mixed
+                                   precision, red-black only)                */
 
 typedef struct smk_mg_params {
   int v_cycles;            /* fixed cycles (tolerance <= 0) or the cap    */

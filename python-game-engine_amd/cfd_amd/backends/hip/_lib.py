@@ -31,6 +31,7 @@ SMK_INNER_F32 = 1
 SMK_COARSEST_SWEEPS = 0
 SMK_COARSEST_OPERATOR = 1
 SMK_COARSEST_EXACT = 2
+SMK_COARSEST_EXACT_1024 = 3
 SMK_ADJ_GATHER = 0
 SMK_ADJ_SCATTER = 1
 

@@ -17,6 +17,7 @@ import ctypes
 import torch
 
 from ._lib import (SMK_COARSE_REDISCRETIZED, SMK_COARSE_REFERENCE, SMK_COARSEST_EXACT,
+                   SMK_COARSEST_EXACT_1024,
                    SMK_COARSEST_OPERATOR, SMK_COARSEST_SWEEPS, SMK_INNER_F32, SMK_INNER_F64,
                    SMK_PREC_MIXED,
                    SMK_PREC_NATIVE, SMK_SMOOTH_GS, SMK_SMOOTH_JACOBI, SMK_SMOOTH_RBSOR, MgParams,
@@ -133,7 +134,7 @@ _SMOOTHERS = {"jacobi": SMK_SMOOTH_JACOBI, "gauss_seidel": SMK_SMOOTH_GS,
 _COARSE = {"reference": SMK_COARSE_REFERENCE, "rediscretized": SMK_COARSE_REDISCRETIZED}
 _PREC = {"native": SMK_PREC_NATIVE, "mixed": SMK_PREC_MIXED}
 _COARSEST = {"sweeps": SMK_COARSEST_SWEEPS, "operator": SMK_COARSEST_OPERATOR,
-             "exact": SMK_COARSEST_EXACT}
+             "exact": SMK_COARSEST_EXACT, "exact_1024": SMK_COARSEST_EXACT_1024}
 _INNER = {"float64": SMK_INNER_F64, "float32": SMK_INNER_F32}
 _GUESS = {"given": 0, "zero": 1}
 

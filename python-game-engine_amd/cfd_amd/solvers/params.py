@@ -73,7 +73,9 @@ class MultigridParams:
     # new: "sweeps" (the reference's), or "operator": the mixed solver's FP32
     # V-cycles apply the coarsest sweeps as their precomputed matrix, or
     # "exact": they apply the coarsest level's exact least-squares solution
-    # operator (the limit of the sweeps) instead
+    # operator (the limit of the sweeps) instead, or "exact_1024": they stop
+    # at the first level of at most 1024 cells and apply its exact operator
+    # (one multi-workgroup product replaces that level and those below)
     coarsest_solve: str = "sweeps"
     # new: "float64" (the reference's expression, as a float32 field's solve
     # evaluates it) or "float32": the mixed solver's FP32 V-cycles smooth in

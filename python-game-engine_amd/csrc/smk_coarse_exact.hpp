@@ -1,6 +1,7 @@
 // smk_coarse_exact.hpp -- host code: the exact solution operator of a
-// multigrid hierarchy's coarsest level (at most 256 cells), applied by the
-// FP32 V-cycles of a mixed-precision solve as one matrix-vector product.
+// multigrid level (at most 1024 cells), applied by the FP32 V-cycles of a
+// mixed-precision solve as one matrix-vector product in place of that level
+// and the levels below it.
 #pragma once
 
 #include <cmath>
@@ -30,11 +31,13 @@ namespace smk {
 // and columns of non-fluid cells zero).  K^+ = (K + P)^-1 - P, P the
 // orthogonal projector on the null space: K + P is positive definite and is
 // inverted by its Cholesky factor in double precision.  Returns 0, or -1
-// when n > 256, the face bits of two neighbours disagree, or the
-// factorization meets a non-positive pivot.
-inline int coarse_exact_operator(const uint8_t* flags, int H, int W, double alpha, float* out) {
+// when n > max_cells (at most COARSE_EXACT_MAX), the face bits of two
+// neighbours disagree, or the factorization meets a non-positive pivot.
+constexpr int COARSE_EXACT_MAX = 1024;
+inline int coarse_exact_operator(const uint8_t* flags, int H, int W, double alpha, float* out,
+                                 int max_cells = 256) {
   const int n = H * W;
-  if (H <= 0 || W <= 0 || n > 256) return -1;
+  if (H <= 0 || W <= 0 || n > max_cells || n > COARSE_EXACT_MAX) return -1;
   std::vector<int> idx(n, -1), cell;
   for (int c = 0; c < n; ++c)
     if (flags[c] & F_FLUID) {
@@ -112,19 +115,26 @@ inline int coarse_exact_operator(const uint8_t* flags, int H, int W, double alph
       K[(size_t)r * m + c] = v / lc;
     }
   }
-  std::vector<double> Z((size_t)m * m, 0.0);
+  // Zt[c * m + r] = (L^-1)[r][c]: column c of L^-1 stored contiguously, so
+  // both loops below run along memory (the sums are taken in the same order
+  // k = c .. r-1 and k = b .. m-1 either way)
+  std::vector<double> Zt((size_t)m * m, 0.0);
   for (int c = 0; c < m; ++c) {  // column c of L^-1: forward substitution
-    Z[(size_t)c * m + c] = 1.0 / K[(size_t)c * m + c];
+    double* z = Zt.data() + (size_t)c * m;
+    z[c] = 1.0 / K[(size_t)c * m + c];
     for (int r = c + 1; r < m; ++r) {
       double v = 0.0;
-      for (int k = c; k < r; ++k) v -= K[(size_t)r * m + k] * Z[(size_t)k * m + c];
-      Z[(size_t)r * m + c] = v / K[(size_t)r * m + r];
+      const double* kr = K.data() + (size_t)r * m;
+      for (int k = c; k < r; ++k) v -= kr[k] * z[k];
+      z[r] = v / K[(size_t)r * m + r];
     }
   }
   for (int a = 0; a < m; ++a)
     for (int b = a; b < m; ++b) {
       double v = 0.0;
-      for (int k = b; k < m; ++k) v += Z[(size_t)k * m + a] * Z[(size_t)k * m + b];
+      const double* za = Zt.data() + (size_t)a * m;
+      const double* zb = Zt.data() + (size_t)b * m;
+      for (int k = b; k < m; ++k) v += za[k] * zb[k];
       const float x = (float)(alpha * (v - proj(a, b)));
       out[(size_t)cell[b] * n + cell[a]] = x;
       out[(size_t)cell[a] * n + cell[b]] = x;

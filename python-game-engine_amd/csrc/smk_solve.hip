@@ -1788,6 +1788,66 @@ __device__ __forceinline__ void coarsest_apply_op(const float* __restrict__ op, 
   __syncthreads();
 }
 
+// x = op b for a level of n <= COARSE_EXACT_MAX cells (op[j * n + i], the
+// level's exact float32 operator) over many workgroups: workgroup g owns rows
+// EX_ROWS g .. + EX_ROWS - 1, and its thread t covers 4 of them
+// (4 (t % 4) .. + 3, one float4 per column) over the column set
+// s = t / 4 (columns j = s, s + EX_SETS, ...).  Every load of a thread is
+// issued before any is used; each (set, row) partial is summed in float over
+// the set's columns in order, then a row's EX_SETS partials in double in set
+// order -- fixed by the code, whatever the launch.
+constexpr int EX_SETS = 64, EX_ROWS = 16, EX_NT = 256;
+constexpr int EX_JMAX = COARSE_EXACT_MAX / EX_SETS;  // columns per set at most
+static_assert(EX_NT == EX_SETS * EX_ROWS / 4, "one thread per (set, 4 rows)");
+
+__global__ __launch_bounds__(EX_NT) void k_apply_exact(const float* __restrict__ op,
+                                                       const float* __restrict__ b,
+                                                       float* __restrict__ x, int n,
+                                                       const int* __restrict__ gate) {
+  if (gate != nullptr && *gate == 0) return;
+  __shared__ float part[EX_SETS][EX_ROWS];
+  const int t = threadIdx.x;
+  const int r4 = 4 * (t % 4), set = t / 4;
+  const int row0 = blockIdx.x * EX_ROWS;
+  const int i0 = row0 + r4;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  if ((n & 3) == 0) {  // rows i0 .. i0 + 3 all inside or all outside; 16-byte aligned
+    float4 m[EX_JMAX];
+    float bj[EX_JMAX];
+#pragma unroll
+    for (int k = 0; k < EX_JMAX; ++k) {
+      const int j = set + k * EX_SETS;
+      const bool ok = j < n && i0 < n;
+      m[k] = ok ? *reinterpret_cast<const float4*>(op + (size_t)j * n + i0)
+                : make_float4(0.f, 0.f, 0.f, 0.f);
+      bj[k] = j < n ? b[j] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < EX_JMAX; ++k) {
+      acc[0] += m[k].x * bj[k];
+      acc[1] += m[k].y * bj[k];
+      acc[2] += m[k].z * bj[k];
+      acc[3] += m[k].w * bj[k];
+    }
+  } else {  // row by row
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + r;
+      for (int j = set; j < n; j += EX_SETS)
+        if (i < n) acc[r] += op[(size_t)j * n + i] * b[j];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) part[set][r4 + r] = acc[r];
+  __syncthreads();
+  if (t < EX_ROWS && row0 + t < n) {
+    double sum = 0.0;
+#pragma unroll 8
+    for (int k = 0; k < EX_SETS; ++k) sum += (double)part[k][t];
+    x[row0 + t] = (float)sum;
+  }
+}
+
 // WAVE (bits): WAVE_COARSEST runs a red-black coarsest level of at most
 // 16 x 16 cells with wave_smooth_level instead of coarsest_wave;
 // WAVE_LEVELS also runs every red-black level of at most 32 x 32 cells, the
@@ -3090,6 +3150,13 @@ struct smk_mg {
   float* cop = nullptr;
   double cop_alpha = 0.0, cop_omega = 0.0;
   int cop_npass = -1;
+  // float32 exact operator of the first level of at most 1024 cells
+  // (SMK_COARSEST_EXACT_1024): its level, and the alpha it was built for
+  // (valid = false after a mask change)
+  float* copw = nullptr;
+  int copw_level = -1;
+  double copw_alpha = 0.0;
+  bool copw_valid = false;
 };
 
 namespace smk {
@@ -3110,6 +3177,10 @@ struct MgRun {
   double rho, dt;
   const float* cop;  // float32 coarsest-level operator for the small kernel (or null)
   bool fast;         // float32 arithmetic in the float32 levels' tile passes
+  // float32 V-cycles: the level whose exact operator copw they apply in
+  // place of it and the levels below (-1: none)
+  int exact_level;
+  const float* copw;
 };
 
 inline double level_omega(const MgRun& run, const Level& L, bool coarsest = false) {
@@ -3253,6 +3324,24 @@ template <typename T>
 int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool zero_init,
               double* norm_partial, int* nparts, hipStream_t s);
 
+// The float32 V-cycle's level l and everything below it in one launch:
+// p32 of level l = the level's exact operator applied to its rhs32
+// (k_apply_exact; the operator is the V-cycle's limit, so no smoothing of
+// these levels is needed).
+inline int mg_apply_exact(smk_mg* mg, int l, const MgRun& run, hipStream_t s) {
+  Level& L = mg->lv[l];
+  const int n = L.H * L.W;
+  if (run.copw == nullptr || n > COARSE_EXACT_MAX)
+    return fail(SMK_EINVAL, "exact level %d (%dx%d): no operator", l, L.H, L.W);
+  float* x = reinterpret_cast<float*>(L.p32[L.cur32]);
+  const float* b = reinterpret_cast<const float*>(L.rhs32);
+  hipEvent_t ev = prof_begin(s);
+  hipLaunchKernelGGL(k_apply_exact, dim3(cdiv(n, EX_ROWS)), dim3(EX_NT), 0, s, run.copw, b, x,
+                     n, run.gate);
+  prof_end(ev, "exact_apply_f32", 4.0 * (double)n * (double)n, s);
+  return check_launch("k_apply_exact");
+}
+
 template <typename T>
 __global__ void k_fill_gated(T* __restrict__ x, long n, T v, const int* __restrict__ gate) {
   if (gate != nullptr && *gate == 0) return;
@@ -3315,6 +3404,10 @@ int mg_vcycle(smk_mg* mg, int l, bool use32, const MgRun& run, double h, bool ze
               double* norm_partial, int* nparts, hipStream_t s) {
   std::vector<Level>& lv = mg->lv;
   if (run.mode == MODE_GS) return mg_vcycle_gs<T>(mg, l, use32, run, h, zero_init, nparts, s);
+  if (use32 && l == run.exact_level) {  // this level and those below: one product
+    if (norm_partial != nullptr) *nparts = -1;
+    return mg_apply_exact(mg, l, run, s);
+  }
   const int from = mg->small_from[use32 ? 1 : 0];
   if (from >= 0 && l >= from) {
     int rc = mg_small<T>(mg, l, use32, run, h, zero_init, s);
@@ -3356,6 +3449,8 @@ inline MgRun make_run(const smk_mg_params* prm, double rho, double dt) {
   run.rediscretize = prm->coarse_scale == SMK_COARSE_REDISCRETIZED;
   run.rho = rho;
   run.dt = dt;
+  run.exact_level = -1;
+  run.copw = nullptr;
   return run;
 }
 
@@ -3573,6 +3668,47 @@ inline int ensure_coarse_op(smk_mg* mg, const MgRun& run, double h, bool exact,
   return SMK_OK;
 }
 
+// The float32 exact operator of the first level of at most 1024 cells
+// (SMK_COARSEST_EXACT_1024), built on the host once per handle, mask state
+// and alpha; *level = -1 and *out = null when no level qualifies or the
+// smoother is not red-black (the V-cycles then keep their coarsest sweeps).
+inline int ensure_wide_op(smk_mg* mg, const MgRun& run, double h, int* level, const float** out,
+                          hipStream_t s) {
+  *level = -1;
+  *out = nullptr;
+  if (run.mode != MODE_RB) return SMK_OK;
+  int l = 0;
+  while (l < mg->nl && (long)mg->lv[l].H * mg->lv[l].W > COARSE_EXACT_MAX) ++l;
+  if (l == mg->nl) return SMK_OK;
+  const Level& C = mg->lv[l];
+  const double alpha = calc_alpha(h * (double)(1L << l), run.rho, run.dt);
+  if (mg->copw == nullptr) {
+    int rc = mg_alloc(reinterpret_cast<void**>(&mg->copw),
+                      (size_t)COARSE_EXACT_MAX * COARSE_EXACT_MAX * sizeof(float));
+    if (rc) return rc;
+    mg->copw_valid = false;
+  }
+  if (!mg->copw_valid || mg->copw_level != l || mg->copw_alpha != alpha) {
+    const int n = C.H * C.W;
+    std::vector<uint8_t> fl((size_t)n);
+    SMK_CHECK_HIP(hipMemcpyAsync(fl.data(), C.flags, (size_t)n, hipMemcpyDeviceToHost, s));
+    SMK_CHECK_HIP(hipStreamSynchronize(s));
+    std::vector<float> op((size_t)n * n);
+    SMK_REQUIRE(coarse_exact_operator(fl.data(), C.H, C.W, alpha, op.data(), COARSE_EXACT_MAX) ==
+                    0,
+                "level %dx%d: no exact operator (inconsistent face flags)", C.H, C.W);
+    SMK_CHECK_HIP(hipMemcpyAsync(mg->copw, op.data(), op.size() * sizeof(float),
+                                 hipMemcpyHostToDevice, s));
+    SMK_CHECK_HIP(hipStreamSynchronize(s));
+    mg->copw_level = l;
+    mg->copw_alpha = alpha;
+    mg->copw_valid = true;
+  }
+  *level = l;
+  *out = mg->copw;
+  return SMK_OK;
+}
+
 // FP32 V-cycles inside FP64 defect correction: r = alpha*div - A p in FP64
 // (with its norm), defect = r/alpha in FP32, e = k FP32 V-cycles from zero,
 // p += e in FP64; repeat until ||r||/||alpha*div|| <= tolerance.
@@ -3585,6 +3721,9 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
   mg_prepare_small<float>(mg, 1, run.mode);
   if (prm->coarsest_solve == SMK_COARSEST_OPERATOR || prm->coarsest_solve == SMK_COARSEST_EXACT) {
     rc = ensure_coarse_op(mg, run, h, prm->coarsest_solve == SMK_COARSEST_EXACT, &run.cop, s);
+    if (rc) return rc;
+  } else if (prm->coarsest_solve == SMK_COARSEST_EXACT_1024) {
+    rc = ensure_wide_op(mg, run, h, &run.exact_level, &run.copw, s);
     if (rc) return rc;
   }
   run.fast = prm->inner_arithmetic == SMK_INNER_F32;
@@ -3957,6 +4096,7 @@ static int mg_build_masks(smk_mg* mg, const uint8_t* fmu, const uint8_t* fmv, hi
     if (!rc) rc = smk_cell_flags(C.fmu, C.fmv, C.H, C.W, C.flags, s);
   }
   mg->cop_npass = -1;  // the coarsest operator follows the coarsest flags
+  mg->copw_valid = false;
   return rc;
 }
 
@@ -3966,6 +4106,7 @@ int smk_mg_refresh_masks(smk_mg* mg, const uint8_t* fmu, const uint8_t* fmv, int
   hipStream_t s = as_stream(stream);
   if (all_levels) return mg_build_masks(mg, fmu, fmv, s);
   if (mg->nl == 1) mg->cop_npass = -1;  // the coarsest level is level 0
+  if (mg->copw_level == 0) mg->copw_valid = false;
   return refresh_cell_flags(fmu, fmv, mg->H, mg->W, mg->lv[0].flags, s);
 }
 
@@ -3990,6 +4131,7 @@ int smk_mg_destroy(smk_mg* mg) {
   (void)hipFree(mg->result);
   (void)hipFree(mg->d_state);
   (void)hipFree(mg->cop);
+  (void)hipFree(mg->copw);
   if (mg->h_ring) (void)hipHostFree(mg->h_ring);
   delete mg;
   return SMK_OK;
@@ -4077,7 +4219,8 @@ int smk_mg_solve(smk_mg* mg, void* p, const void* div, double rho, double h, dou
               "smk_mg_solve: bad coarse_scale");
   SMK_REQUIRE(prm->coarsest_solve == SMK_COARSEST_SWEEPS ||
                   prm->coarsest_solve == SMK_COARSEST_OPERATOR ||
-                  prm->coarsest_solve == SMK_COARSEST_EXACT,
+                  prm->coarsest_solve == SMK_COARSEST_EXACT ||
+                  prm->coarsest_solve == SMK_COARSEST_EXACT_1024,
               "smk_mg_solve: bad coarsest_solve %d", prm->coarsest_solve);
   hipStream_t s = as_stream(stream);
   if (prm->precision == SMK_PREC_MIXED) {

@@ -31,8 +31,9 @@ F_FLUID, F_LEFT, F_RIGHT, F_DOWN, F_UP = 1, 2, 4, 8, 16
 
 SHIM = r"""
 #include "smk_coarse_exact.hpp"
-extern "C" int coarse_exact(const uint8_t* flags, int H, int W, double alpha, float* out) {
-  return smk::coarse_exact_operator(flags, H, W, alpha, out);
+extern "C" int coarse_exact(const uint8_t* flags, int H, int W, double alpha, float* out,
+                            int max_cells) {
+  return smk::coarse_exact_operator(flags, H, W, alpha, out, max_cells);
 }
 """
 
@@ -47,14 +48,14 @@ def exact(tmp_path_factory):
     lib = ctypes.CDLL(str(so))
     lib.coarse_exact.restype = ctypes.c_int
     lib.coarse_exact.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                 ctypes.c_void_p]
+                                 ctypes.c_void_p, ctypes.c_int]
 
-    def run(flags):
+    def run(flags, max_cells=1024):
         H, W = flags.shape
         n = H * W
         fl = np.ascontiguousarray(flags, dtype=np.uint8)
         out = np.zeros(n * n, dtype=np.float32)
-        rc = lib.coarse_exact(fl.ctypes.data, H, W, ALPHA, out.ctypes.data)
+        rc = lib.coarse_exact(fl.ctypes.data, H, W, ALPHA, out.ctypes.data, max_cells)
         return rc, out.reshape(n, n)  # out[j, i]: cell i for the unit rhs j
 
     return run
@@ -122,6 +123,19 @@ def cases():
     rag[0, :] = rag[-1, :] = rag[:, 0] = rag[:, -1] = True
     rag[4:6, 3:5] = True
     out["ragged_10x12"] = rag
+    # levels of up to 1024 cells (coarsest_solve "exact_1024")
+    big = np.zeros((32, 32), dtype=bool)
+    big[0, :] = big[-1, :] = big[:, 0] = big[:, -1] = True
+    out["open_32"] = big.copy()
+    big[9:17, 8:13] = True
+    big[21, 18:27] = True
+    big[25, 5] = True
+    out["obstacles_32"] = big.copy()
+    rooms = np.zeros((24, 40), dtype=bool)  # 960 cells
+    rooms[0, :] = rooms[-1, :] = rooms[:, 0] = rooms[:, -1] = True
+    rooms[:, 17] = True
+    rooms[6:9, 25:30] = True
+    out["two_rooms_24x40"] = rooms
     return out
 
 
@@ -171,9 +185,9 @@ def test_least_squares_least_norm(exact, name, stale):
     rc, op = exact(flags_of(fmu, fmv, is_fluid))
     assert rc == 0
     parts = neumann_parts(fmu, fmv, is_fluid)
-    if name in ("open", "obstacles", "ragged_10x12") and stale is None:
+    if name in ("open", "obstacles", "ragged_10x12", "open_32", "obstacles_32") and stale is None:
         assert len(parts) == 1
-    if name == "two_rooms" and stale is None:
+    if name in ("two_rooms", "two_rooms_24x40") and stale is None:
         assert len(parts) == 2
     if stale is not None:  # the stale cells' open faces leak: a nonsingular part
         assert len(parts) < len(neumann_parts(fmu, fmv, O.is_fluid_from_masks(fmu, fmv)))
@@ -209,4 +223,8 @@ def test_rejects_one_sided_faces_and_large_levels(exact):
     fl[1, 1] = F_FLUID | F_RIGHT
     fl[1, 2] = F_FLUID  # no F_LEFT back
     assert exact(fl)[0] == -1
-    assert exact(np.zeros((17, 16), dtype=np.uint8))[0] == -1
+    assert exact(np.zeros((17, 16), dtype=np.uint8), 256)[0] == -1
+    assert exact(np.zeros((16, 16), dtype=np.uint8), 256)[0] == 0
+    assert exact(np.zeros((33, 32), dtype=np.uint8), 1024)[0] == -1
+    assert exact(np.zeros((32, 32), dtype=np.uint8), 2048)[0] == 0  # the cap still holds
+    assert exact(np.zeros((33, 32), dtype=np.uint8), 2048)[0] == -1

@@ -178,3 +178,79 @@ def test_exact_operator_cache_follows_the_mode():
         got, _ = solve(256, 256, mg, coarsest_solve=cs, **kw)
         np.testing.assert_array_equal(got, fresh[cs])
     assert not np.array_equal(fresh["exact"], fresh["operator"])
+
+
+# coarsest_solve = "exact_1024": the FP32 V-cycles stop at the first level of
+# at most 1024 cells and apply its exact operator there (k_apply_exact, many
+# workgroups); the levels below are not visited
+
+def test_exact_1024_converges_like_exact():
+    """256x256, 5 levels: the cycles stop at 32x32.  At the same tolerance the
+    solve converges in no more refinements than the 16x16 exact level, to a
+    result as close to the tightly converged solution."""
+    kw = dict(num_levels=5, v_cycles=200, inner_cycles=2, block=False)
+    tight, it = solve(256, 256, tolerance=1e-13, **dict(kw, precision="native"))
+    assert it["converged"]
+    res = {}
+    for cs in ("exact", "exact_1024"):
+        got, info = solve(256, 256, tolerance=1e-8, coarsest_solve=cs, **kw)
+        assert info["converged"] and info["rel_residual"] <= 1e-8, (cs, info)
+        res[cs] = (up_to_constant(got, tight), info)
+    assert res["exact_1024"][1]["refinements"] <= res["exact"][1]["refinements"], res
+    assert res["exact_1024"][0] <= 2.0 * res["exact"][0] + 1e-9 * np.abs(tight).max(), res
+
+
+@pytest.mark.parametrize("shape", [(200, 136, 4), (192, 192, 4)], ids=["200x136", "192x192"])
+def test_exact_1024_ragged_level(shape):
+    """200x136 and 192x192 in 4 levels: the first level of at most 1024 cells
+    is the coarsest, 25x17 (425 cells, not a multiple of 4: the product's
+    row-by-row path) or 24x24 (576).  Solving it exactly instead of by its 50
+    sweeps converges in fewer refinements (measured 4 and 3 against 12), to a
+    result at least as close to the tightly converged solution as the FP64
+    V-cycles' own at the same tolerance (whose error is the smooth component
+    the coarsest sweeps leave behind)."""
+    h, w, levels = shape
+    kw = dict(num_levels=levels, v_cycles=400, inner_cycles=2, block=False)
+    tight, it = solve(h, w, tolerance=1e-13, **dict(kw, precision="native"))
+    ref, iref = solve(h, w, tolerance=1e-8, **dict(kw, precision="native"))
+    swp, isw = solve(h, w, tolerance=1e-8, coarsest_solve="sweeps", **kw)
+    got, info = solve(h, w, tolerance=1e-8, coarsest_solve="exact_1024", **kw)
+    assert it["converged"] and iref["converged"], (it, iref)
+    assert info["converged"] and info["rel_residual"] <= 1e-8, info
+    assert info["refinements"] < isw["refinements"], (info, isw)
+    scale = np.abs(tight).max()
+    assert up_to_constant(got, tight) <= max(up_to_constant(ref, tight), 1e-7 * scale)
+
+
+def test_exact_1024_single_level_is_a_direct_solve():
+    """A 32x32 grid of one level: the V-cycle is the 1024x1024 operator itself,
+    so each FP64 refinement gains float32's accuracy."""
+    got, info = solve(32, 32, coarsest_solve="exact_1024", num_levels=1, v_cycles=20,
+                      inner_cycles=1, tolerance=1e-11)
+    assert info["converged"] and info["rel_residual"] <= 1e-11, info
+    assert info["refinements"] <= 4, info
+
+
+def test_exact_1024_is_red_black_only():
+    """With the Jacobi smoother the option keeps the coarsest sweeps, bit for
+    bit."""
+    kw = dict(num_levels=5, v_cycles=3, inner_cycles=1, smoother_type="jacobi")
+    ref, _ = solve(256, 256, coarsest_solve="sweeps", **kw)
+    got, _ = solve(256, 256, coarsest_solve="exact_1024", **kw)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_exact_1024_cache_follows_mode_and_masks():
+    """One handle alternating the two exact kinds and two mask sets (every
+    level derived again from the masks passed in, rebuild_masks) gives each
+    case's fresh-handle result bit for bit."""
+    from cfd_amd.backends.hip import solvers
+
+    mg = solvers.MultigridSolver()
+    kw = dict(num_levels=5, v_cycles=3, inner_cycles=1, rebuild_masks=True)
+    cases = [("exact_1024", True), ("exact", True), ("exact_1024", False), ("exact_1024", True)]
+    fresh = {c: solve(256, 256, coarsest_solve=c[0], block=c[1], **kw)[0] for c in set(cases)}
+    for cs, block in cases:
+        got, _ = solve(256, 256, mg, coarsest_solve=cs, block=block, **kw)
+        np.testing.assert_array_equal(got, fresh[(cs, block)])
+    assert not np.array_equal(fresh[("exact_1024", True)], fresh[("exact", True)])

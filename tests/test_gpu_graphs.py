@@ -31,9 +31,9 @@ def _same(a, b, where):
         assert torch.equal(x, y), f"{where}: {f} differs (max {float((x - y).abs().max())})"
 
 
-@pytest.mark.parametrize("overrides", [None, {"precision": "mixed", "inner_cycles": 2,
-                                              "coarsest_solve": "exact",
-                                              "inner_arithmetic": "float32"}])
+@pytest.mark.parametrize("overrides", [None] + [
+    {"precision": "mixed", "inner_cycles": 2, "coarsest_solve": cs, "inner_arithmetic": "float32"}
+    for cs in ("exact", "exact_1024")])
 def test_graph_stages_match_eager_steps(overrides):
     sc = scenes.plume_2048_mg(N=256, steps=8)
     a, b = _pair(sc, overrides)

@@ -137,9 +137,10 @@ def test_mixed_rbsor_matches_fp64_to_tolerance(N):
     assert np.abs(a - b).max() <= 1e-5 * np.abs(a).max()
 
 
-# the bench line's headline variant: two FP32 V-cycles per FP64 refinement,
-# the coarsest level's exact operator, float32 pass arithmetic
-HEADLINE_MIXED = {"precision": "mixed", "inner_cycles": 2, "coarsest_solve": "exact",
+# the bench line's headline variant: two FP32 V-cycles per FP64 refinement
+# down to the 32x32 level, solved there by its exact operator, float32 pass
+# arithmetic
+HEADLINE_MIXED = {"precision": "mixed", "inner_cycles": 2, "coarsest_solve": "exact_1024",
                   "inner_arithmetic": "float32"}
 
 

@@ -166,14 +166,16 @@ def test_tile_variant_tolerance_solve(name):
 FAST_VARIANTS = [v for v in TILE_VARIANTS if v != "reg64"]
 
 
+@pytest.mark.parametrize("coarsest", ["operator", "exact_1024"])
 @pytest.mark.parametrize("which", ["fine", "coarse_small"])
 @pytest.mark.parametrize("name", FAST_VARIANTS)
-def test_tile_variant_mixed_float32_arithmetic(which, name):
+def test_tile_variant_mixed_float32_arithmetic(which, name, coarsest):
     """FP32 V-cycles in float32 arithmetic inside FP64 refinement (the
-    headline configuration's solve): every variant the autotuner may pick
-    gives the default variant's pressure and cycle counts, bit for bit."""
+    headline configuration's solve, with the coarsest level's operator or
+    the 32x32 level's exact one): every variant the autotuner may pick gives
+    the default variant's pressure and cycle counts, bit for bit."""
     kw = dict(v_cycles=60, tolerance=1e-9, precision="mixed", inner_cycles=2,
-              inner_arithmetic="float32", coarsest_solve="operator")
+              inner_arithmetic="float32", coarsest_solve=coarsest)
     want, winfo = run_mg("float64", **kw)
     with variant("float32", which, name):
         got, info = run_mg("float64", **kw)
