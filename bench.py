@@ -155,6 +155,7 @@ TRAFFIC_FAMILIES = [
     ("k_smooth_tile64_r<double", "rb_smooth_coarse_f64"),
     ("k_smooth_tile64_d<double", "rb_smooth_coarse_f64"),
     ("k_vcycle_small<float", "vcycle_small_f32"), ("k_vcycle_small<double", "vcycle_small_f64"),
+    ("k_apply_exact", "exact_apply_f32"),
     ("k_residual_norm<double", "residual_norm_f64"), ("k_correct_defect<double", "correct_defect_fp64"),
     ("k_advect_velocity<double", "advect_velocity_f64"), ("k_advect_scalar<double", "advect_scalar_f64"),
     ("k_pressure_gradient<double", "pressure_gradient_f64"), ("k_divergence<double", "divergence_f64"),
