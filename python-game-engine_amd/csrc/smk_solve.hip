@@ -1788,8 +1788,10 @@ __device__ __forceinline__ void coarsest_apply_op(const float* __restrict__ op, 
   __syncthreads();
 }
 
-// x = op b for a level of n <= COARSE_EXACT_MAX cells (op[j * n + i], the
-// level's exact float32 operator) over many workgroups: workgroup g owns rows
+// x = op (alpha b) for a level of n <= COARSE_EXACT_MAX cells (op[j * n + i],
+// the level's exact float32 operator K^+ without its alpha, so that one build
+// serves every dt; alpha b is formed in float32) over many workgroups:
+// workgroup g owns rows
 // EX_ROWS g .. + EX_ROWS - 1, and its thread t covers 4 of them
 // (4 (t % 4) .. + 3, one float4 per column) over the column set
 // s = t / 4 (columns j = s, s + EX_SETS, ...).  Every load of a thread is
@@ -1803,6 +1805,7 @@ static_assert(EX_NT == EX_SETS * EX_ROWS / 4, "one thread per (set, 4 rows)");
 __global__ __launch_bounds__(EX_NT) void k_apply_exact(const float* __restrict__ op,
                                                        const float* __restrict__ b,
                                                        float* __restrict__ x, int n,
+                                                       float alpha,
                                                        const int* __restrict__ gate) {
   if (gate != nullptr && *gate == 0) return;
   __shared__ float part[EX_SETS][EX_ROWS];
@@ -1820,7 +1823,7 @@ __global__ __launch_bounds__(EX_NT) void k_apply_exact(const float* __restrict__
       const bool ok = j < n && i0 < n;
       m[k] = ok ? *reinterpret_cast<const float4*>(op + (size_t)j * n + i0)
                 : make_float4(0.f, 0.f, 0.f, 0.f);
-      bj[k] = j < n ? b[j] : 0.f;
+      bj[k] = j < n ? alpha * b[j] : 0.f;
     }
 #pragma unroll
     for (int k = 0; k < EX_JMAX; ++k) {
@@ -1834,7 +1837,7 @@ __global__ __launch_bounds__(EX_NT) void k_apply_exact(const float* __restrict__
     for (int r = 0; r < 4; ++r) {
       const int i = i0 + r;
       for (int j = set; j < n; j += EX_SETS)
-        if (i < n) acc[r] += op[(size_t)j * n + i] * b[j];
+        if (i < n) acc[r] += op[(size_t)j * n + i] * (alpha * b[j]);
     }
   }
 #pragma unroll
@@ -3150,12 +3153,11 @@ struct smk_mg {
   float* cop = nullptr;
   double cop_alpha = 0.0, cop_omega = 0.0;
   int cop_npass = -1;
-  // float32 exact operator of the first level of at most 1024 cells
-  // (SMK_COARSEST_EXACT_1024): its level, and the alpha it was built for
-  // (valid = false after a mask change)
+  // float32 exact operator (without alpha) of the first level of at most 1024
+  // cells (SMK_COARSEST_EXACT_1024): its level (valid = false after a mask
+  // change)
   float* copw = nullptr;
   int copw_level = -1;
-  double copw_alpha = 0.0;
   bool copw_valid = false;
 };
 
@@ -3177,10 +3179,12 @@ struct MgRun {
   double rho, dt;
   const float* cop;  // float32 coarsest-level operator for the small kernel (or null)
   bool fast;         // float32 arithmetic in the float32 levels' tile passes
-  // float32 V-cycles: the level whose exact operator copw they apply in
-  // place of it and the levels below (-1: none)
+  // float32 V-cycles: the level whose exact operator copw (without alpha)
+  // they apply in place of it and the levels below (-1: none), and that
+  // level's alpha
   int exact_level;
   const float* copw;
+  double exact_alpha;
 };
 
 inline double level_omega(const MgRun& run, const Level& L, bool coarsest = false) {
@@ -3337,7 +3341,7 @@ inline int mg_apply_exact(smk_mg* mg, int l, const MgRun& run, hipStream_t s) {
   const float* b = reinterpret_cast<const float*>(L.rhs32);
   hipEvent_t ev = prof_begin(s);
   hipLaunchKernelGGL(k_apply_exact, dim3(cdiv(n, EX_ROWS)), dim3(EX_NT), 0, s, run.copw, b, x,
-                     n, run.gate);
+                     n, (float)run.exact_alpha, run.gate);
   prof_end(ev, "exact_apply_f32", 4.0 * (double)n * (double)n, s);
   return check_launch("k_apply_exact");
 }
@@ -3451,6 +3455,7 @@ inline MgRun make_run(const smk_mg_params* prm, double rho, double dt) {
   run.dt = dt;
   run.exact_level = -1;
   run.copw = nullptr;
+  run.exact_alpha = 0.0;
   return run;
 }
 
@@ -3668,12 +3673,13 @@ inline int ensure_coarse_op(smk_mg* mg, const MgRun& run, double h, bool exact,
   return SMK_OK;
 }
 
-// The float32 exact operator of the first level of at most 1024 cells
-// (SMK_COARSEST_EXACT_1024), built on the host once per handle, mask state
-// and alpha; *level = -1 and *out = null when no level qualifies or the
-// smoother is not red-black (the V-cycles then keep their coarsest sweeps).
+// The float32 exact operator K^+ (alpha applied by the product) of the first
+// level of at most 1024 cells (SMK_COARSEST_EXACT_1024), built on the host
+// once per handle and mask state; *level = -1 and *out = null when no level
+// qualifies or the smoother is not red-black (the V-cycles then keep their
+// coarsest sweeps).  *alpha = that level's alpha for this solve.
 inline int ensure_wide_op(smk_mg* mg, const MgRun& run, double h, int* level, const float** out,
-                          hipStream_t s) {
+                          double* alpha_out, hipStream_t s) {
   *level = -1;
   *out = nullptr;
   if (run.mode != MODE_RB) return SMK_OK;
@@ -3681,27 +3687,25 @@ inline int ensure_wide_op(smk_mg* mg, const MgRun& run, double h, int* level, co
   while (l < mg->nl && (long)mg->lv[l].H * mg->lv[l].W > COARSE_EXACT_MAX) ++l;
   if (l == mg->nl) return SMK_OK;
   const Level& C = mg->lv[l];
-  const double alpha = calc_alpha(h * (double)(1L << l), run.rho, run.dt);
+  *alpha_out = calc_alpha(h * (double)(1L << l), run.rho, run.dt);
   if (mg->copw == nullptr) {
     int rc = mg_alloc(reinterpret_cast<void**>(&mg->copw),
                       (size_t)COARSE_EXACT_MAX * COARSE_EXACT_MAX * sizeof(float));
     if (rc) return rc;
     mg->copw_valid = false;
   }
-  if (!mg->copw_valid || mg->copw_level != l || mg->copw_alpha != alpha) {
+  if (!mg->copw_valid || mg->copw_level != l) {
     const int n = C.H * C.W;
     std::vector<uint8_t> fl((size_t)n);
     SMK_CHECK_HIP(hipMemcpyAsync(fl.data(), C.flags, (size_t)n, hipMemcpyDeviceToHost, s));
     SMK_CHECK_HIP(hipStreamSynchronize(s));
     std::vector<float> op((size_t)n * n);
-    SMK_REQUIRE(coarse_exact_operator(fl.data(), C.H, C.W, alpha, op.data(), COARSE_EXACT_MAX) ==
-                    0,
+    SMK_REQUIRE(coarse_exact_operator(fl.data(), C.H, C.W, 1.0, op.data(), COARSE_EXACT_MAX) == 0,
                 "level %dx%d: no exact operator (inconsistent face flags)", C.H, C.W);
     SMK_CHECK_HIP(hipMemcpyAsync(mg->copw, op.data(), op.size() * sizeof(float),
                                  hipMemcpyHostToDevice, s));
     SMK_CHECK_HIP(hipStreamSynchronize(s));
     mg->copw_level = l;
-    mg->copw_alpha = alpha;
     mg->copw_valid = true;
   }
   *level = l;
@@ -3723,7 +3727,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     rc = ensure_coarse_op(mg, run, h, prm->coarsest_solve == SMK_COARSEST_EXACT, &run.cop, s);
     if (rc) return rc;
   } else if (prm->coarsest_solve == SMK_COARSEST_EXACT_1024) {
-    rc = ensure_wide_op(mg, run, h, &run.exact_level, &run.copw, s);
+    rc = ensure_wide_op(mg, run, h, &run.exact_level, &run.copw, &run.exact_alpha, s);
     if (rc) return rc;
   }
   run.fast = prm->inner_arithmetic == SMK_INNER_F32;

@@ -48,14 +48,14 @@ def up_to_constant(a, b):
     return np.abs(d - d.mean()).max()
 
 
-def solve(H, W, solver=None, block=True, **kw):
+def solve(H, W, solver=None, block=True, dt=0.005, **kw):
     from cfd_amd.backends.hip import solvers
 
     d, fmu, fmv = problem(H, W, block=block)
     out = torch.zeros((H, W), dtype=torch.float64, device="cuda")
     solver = solver or solvers.MultigridSolver()
     solver(out, torch.from_numpy(d).cuda(), torch.from_numpy(fmu).cuda(),
-           torch.from_numpy(fmv).cuda(), 1.0, 1.0 / H, 0.005, **dict(MG, **kw))
+           torch.from_numpy(fmv).cuda(), 1.0, 1.0 / H, dt, **dict(MG, **kw))
     torch.cuda.synchronize()
     return out.cpu().numpy(), dict(solvers.LAST_SOLVE_INFO)
 
@@ -254,3 +254,22 @@ def test_exact_1024_cache_follows_mode_and_masks():
         got, _ = solve(256, 256, mg, coarsest_solve=cs, block=block, **kw)
         np.testing.assert_array_equal(got, fresh[(cs, block)])
     assert not np.array_equal(fresh[("exact_1024", True)], fresh[("exact", True)])
+
+
+def test_exact_1024_operator_serves_every_dt():
+    """The 1024-cell operator is built without alpha (the product applies the
+    level's alpha), so one handle solving at two time steps gives each time
+    step's fresh-handle result bit for bit, and a tolerance solve at the
+    second time step still converges."""
+    from cfd_amd.backends.hip import solvers
+
+    mg = solvers.MultigridSolver()
+    kw = dict(num_levels=5, v_cycles=3, inner_cycles=1, coarsest_solve="exact_1024", block=False)
+    fresh = {dt: solve(256, 256, dt=dt, **kw)[0] for dt in (0.005, 0.02)}
+    for dt in (0.005, 0.02, 0.005):
+        got, _ = solve(256, 256, mg, dt=dt, **kw)
+        np.testing.assert_array_equal(got, fresh[dt])
+    assert not np.array_equal(fresh[0.005], fresh[0.02])
+    got, info = solve(256, 256, mg, dt=0.02, **dict(kw, v_cycles=60, inner_cycles=2,
+                                                      tolerance=1e-8))
+    assert info["converged"] and info["rel_residual"] <= 1e-8, info
