@@ -2233,15 +2233,29 @@ __global__ __launch_bounds__(1024) void k_check(const double* __restrict__ parti
                                                 int cap, int inner,
                                                 RingSlot* __restrict__ host_out, unsigned seq) {
   __shared__ double red[1024 / 64];
-  if (st->active == 0) return;
+  // every load is issued before any result is used (the state, the rhs
+  // norm, the partials in batches), so the check waits on memory once; a
+  // thread adds its partials in the same order as one after another
+  constexpr int B = 16;
+  const int active = st->active, c = st->cycles;
+  const double bb = *b2;
   double acc = 0.0;
-  for (int k = threadIdx.x; k < n; k += 1024) acc += partial[k];
+  for (int k0 = threadIdx.x; k0 < n; k0 += B * 1024) {
+    double v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int k = k0 + b * 1024;
+      v[b] = partial[k < n ? k : n - 1];
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+      if (k0 + b * 1024 < n) acc += v[b];
+  }
+  if (active == 0) return;
   const double tot = block_sum<1024>(acc, red);
   if (threadIdx.x != 0) return;
-  const double bb = *b2;
   const double rel = bb > 0.0 ? sqrt(tot / bb) : 0.0;
   st->rel = rel;
-  const int c = st->cycles;
   const bool done = (tol > 0.0 ? rel <= tol : c >= cap) || c * inner >= cap;
   SolveState out{done ? 0 : 1, done ? c : c + 1, rel};
   *st = out;
