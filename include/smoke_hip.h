@@ -149,6 +149,14 @@ int smk_emit_scalar(void* field, int H, int W, double value, double dt,
  * field[y0:y1, x0:x1] = value (mode 0) or += value (mode 1). */
 int smk_box_source(void* field, int H, int W, int x0, int y0, int x1, int y1,
                    double value, int mode, int dtype, void* stream);
+/* n box sources on one field, applied in list order (no reference
+ * counterpart; the plume workloads' emitters): rects[4k .. 4k+3] = x0, y0,
+ * x1, y1 of box k, values[k], modes[k] as in smk_box_source.  When the
+ * clamped boxes are pairwise disjoint (and n <= 8) one launch applies all of
+ * them (each cell is written by at most one box, so the order cannot
+ * matter); otherwise they are applied one launch after another. */
+int smk_box_sources(void* field, int H, int W, int n, const int* rects,
+                    const double* values, const int* modes, int dtype, void* stream);
 
 /* -------------------------------------------------------------- solvers */
 

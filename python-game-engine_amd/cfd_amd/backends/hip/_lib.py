@@ -135,6 +135,7 @@ SIGNATURES = {
                                      _i, _vp]),
     "smk_emit_scalar": (_i, [_vp, _i, _i, _d, _d, _d, _d, _d, _i, _i, _vp]),
     "smk_box_source": (_i, [_vp, _i, _i, _i, _i, _i, _i, _d, _i, _i, _vp]),
+    "smk_box_sources": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
     "smk_solver_workspace_bytes": (_sz, [_i, _i, _i]),
     "smk_jacobi_solve": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _vp, _sz, _i, _vp]),
     "smk_jacobi_solve_noalloc": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _d, _d, _i, _i, _vp]),

@@ -13,6 +13,7 @@ wrong shape there reads out of bounds).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import torch
@@ -326,4 +327,21 @@ def box_source(field, x0, y0, x1, y1, value, mode="set"):
     H, W = field.shape
     call("smk_box_source", _p(field), H, W, int(x0), int(y0), int(x1), int(y1), float(value),
          0 if mode == "set" else 1, dtype_code(field), stream_of(field))
+
+
+def box_sources(field, boxes):
+    """box_source for each (x0, y0, x1, y1, value, mode) of ``boxes`` in list
+    order; pairwise-disjoint boxes (after clamping to the field) run in one
+    launch."""
+    _field(field, "field")
+    n = len(boxes)
+    for b in boxes:
+        if b[5] not in ("set", "add"):
+            raise ValueError("mode must be 'set' or 'add'")
+    H, W = field.shape
+    rects = (ctypes.c_int * max(4 * n, 1))(*[int(v) for b in boxes for v in b[:4]])
+    values = (ctypes.c_double * max(n, 1))(*[float(b[4]) for b in boxes])
+    modes = (ctypes.c_int * max(n, 1))(*[0 if b[5] == "set" else 1 for b in boxes])
+    call("smk_box_sources", _p(field), H, W, n, ctypes.addressof(rects),
+         ctypes.addressof(values), ctypes.addressof(modes), dtype_code(field), stream_of(field))
 

@@ -34,6 +34,7 @@ import torch
 from cfd_amd.backends.hip import operators as hip
 from cfd_amd.backends.hip import solvers as hip_solvers
 from cfd_amd.common.config import Backend, SimulationConfig
+from cfd_amd.common.emitters import BoxEmitter
 from cfd_amd.operators.advection import (advect_scalar_field, advect_velocity_u,
                                          advect_velocity_v)
 from cfd_amd.operators.boundaries import enforce_boundary_conditions
@@ -338,8 +339,23 @@ class SmokeSimulation2D:
     # --------------------------------------------------------- components
     def _apply_emitters(self, dt):
         temp = self.temperature if self._temperature_active() else None
+        run = []  # consecutive enabled box emitters: their boxes in one call
+
+        def flush():
+            if len(run) > 1:
+                hip.box_sources(self.density, [(b.x0, b.y0, b.x1, b.y1, b.value, b.mode)
+                                               for b in run])
+            elif run:
+                run[0].emit(self.density, self.velocity_u, self.velocity_v, temp, dt)
+            run.clear()
+
         for e in self.emitters:
+            if type(e) is BoxEmitter and e.enabled and self.density is not None:
+                run.append(e)
+                continue
+            flush()
             e.emit(self.density, self.velocity_u, self.velocity_v, temp, dt)
+        flush()
 
     def _apply_forces(self, dt):
         """simulation.py:273-277 (density_coefficient fixed at 1.0 there)."""

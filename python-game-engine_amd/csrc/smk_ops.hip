@@ -556,6 +556,25 @@ __global__ void k_box(T* __restrict__ f, int W, int x0, int y0, int x1, int y1, 
   f[o] = mode == 0 ? value : f[o] + value;
 }
 
+// Up to BOX_MAX pairwise-disjoint boxes in one launch: blockIdx.z is the
+// box, each block a 64 x 4 piece of it (the grid spans the largest box).
+constexpr int BOX_MAX = 8;
+struct BoxList {
+  int x0[BOX_MAX], y0[BOX_MAX], x1[BOX_MAX], y1[BOX_MAX], mode[BOX_MAX];
+  double value[BOX_MAX];
+};
+
+template <typename T>
+__global__ void k_boxes(T* __restrict__ f, int W, const BoxList b) {
+  const int k = blockIdx.z;
+  const int x = b.x0[k] + blockIdx.x * BX + threadIdx.x;
+  const int y = b.y0[k] + blockIdx.y * BY + threadIdx.y;
+  if (x >= b.x1[k] || y >= b.y1[k]) return;
+  const size_t o = (size_t)y * W + x;
+  const T value = (T)b.value[k];
+  f[o] = b.mode[k] == 0 ? value : f[o] + value;
+}
+
 // ------------------------------------------------------ batched transport
 // The same per-face / per-cell arithmetic as the kernels above, for many
 // scenes in one persistent launch: workgroups walk a packed job list of
@@ -1075,6 +1094,51 @@ int smk_box_source(void* field, int H, int W, int x0, int y0, int x1, int y1, do
                hipLaunchKernelGGL(k_box<float>, g, dim3(BX, BY), 0, s, (float*)field, W, x0, y0,
                                   x1, y1, (float)value, mode));
   return check_launch("k_box");
+}
+
+int smk_box_sources(void* field, int H, int W, int n, const int* rects, const double* values,
+                    const int* modes, int dtype, void* stream) {
+  SMK_REQUIRE(field && H > 0 && W > 0 && n >= 0, "smk_box_sources: bad arguments");
+  SMK_REQUIRE(n == 0 || (rects && values && modes), "smk_box_sources: null box list");
+  BoxList b{};
+  int m = 0, maxw = 0, maxh = 0;
+  bool fused = n <= BOX_MAX;
+  for (int k = 0; k < n; ++k) {
+    SMK_REQUIRE(modes[k] == 0 || modes[k] == 1,
+                "smk_box_sources: mode must be 0 (set) or 1 (add)");
+    const int x0 = rects[4 * k] < 0 ? 0 : rects[4 * k];
+    const int y0 = rects[4 * k + 1] < 0 ? 0 : rects[4 * k + 1];
+    const int x1 = rects[4 * k + 2] > W ? W : rects[4 * k + 2];
+    const int y1 = rects[4 * k + 3] > H ? H : rects[4 * k + 3];
+    if (x1 <= x0 || y1 <= y0 || !fused) continue;
+    for (int q = 0; q < m && fused; ++q)  // overlap with a box already listed
+      if (x0 < b.x1[q] && b.x0[q] < x1 && y0 < b.y1[q] && b.y0[q] < y1) fused = false;
+    if (!fused) continue;
+    b.x0[m] = x0;
+    b.y0[m] = y0;
+    b.x1[m] = x1;
+    b.y1[m] = y1;
+    b.mode[m] = modes[k];
+    b.value[m] = values[k];
+    maxw = x1 - x0 > maxw ? x1 - x0 : maxw;
+    maxh = y1 - y0 > maxh ? y1 - y0 : maxh;
+    ++m;
+  }
+  if (!fused) {  // overlapping boxes: one after another, in list order
+    for (int k = 0; k < n; ++k) {
+      int rc = smk_box_source(field, H, W, rects[4 * k], rects[4 * k + 1], rects[4 * k + 2],
+                              rects[4 * k + 3], values[k], modes[k], dtype, stream);
+      if (rc) return rc;
+    }
+    return SMK_OK;
+  }
+  if (m == 0) return SMK_OK;
+  dim3 g(cdiv(maxw, BX), cdiv(maxh, BY), m);
+  auto s = as_stream(stream);
+  SMK_DISPATCH(dtype,
+               hipLaunchKernelGGL(k_boxes<double>, g, dim3(BX, BY), 0, s, (double*)field, W, b),
+               hipLaunchKernelGGL(k_boxes<float>, g, dim3(BX, BY), 0, s, (float*)field, W, b));
+  return check_launch("k_boxes");
 }
 
 }  // extern "C"

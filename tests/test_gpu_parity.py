@@ -187,6 +187,41 @@ def test_box_source(hip):
     np.testing.assert_array_equal(host(out), ref)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("case", ["disjoint", "overlapping", "clamped_and_empty", "many"])
+def test_box_sources_equal_one_box_after_another(hip, dtype, case):
+    """Several boxes in one call (one launch when the clamped boxes are
+    pairwise disjoint, else one launch per box) equal the numpy boxes applied
+    in list order, bit for bit, in set and add modes."""
+    ops, _ = hip
+    boxes = {
+        "disjoint": [(2, 3, 10, 9, 1.25, "set"), (20, 3, 31, 12, 0.5, "add"),
+                     (12, 20, 40, 35, -2.0, "set"), (41, 0, 50, 40, 0.75, "add")],
+        # the second box overlaps the first: order matters for the add
+        "overlapping": [(5, 5, 20, 20, 1.0, "set"), (10, 10, 30, 30, 0.25, "add"),
+                        (15, 0, 25, 8, 3.0, "set")],
+        "clamped_and_empty": [(-5, -5, 4, 6, 2.0, "set"), (45, 35, 70, 90, 0.5, "add"),
+                              (10, 10, 10, 20, 9.0, "set"), (30, 20, 25, 30, 9.0, "set")],
+        # more boxes than one launch takes: one after another
+        "many": [(4 * k, 2 * k, 4 * k + 3, 2 * k + 2, 0.1 * k, "add") for k in range(10)],
+    }[case]
+    f = np.random.default_rng(7).random((40, 50)).astype(dtype)
+    out = torch.from_numpy(f.copy()).cuda()
+    ops.box_sources(out, boxes)
+    ref = f.copy()
+    for x0, y0, x1, y1, v, mode in boxes:
+        x0, y0, x1, y1 = max(x0, 0), max(y0, 0), min(x1, 50), min(y1, 40)
+        if x1 <= x0 or y1 <= y0:
+            continue
+        if mode == "set":
+            ref[y0:y1, x0:x1] = dtype(v)
+        else:
+            ref[y0:y1, x0:x1] += dtype(v)
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    with pytest.raises(ValueError, match="mode must be"):
+        ops.box_sources(out, [(0, 0, 2, 2, 1.0, "max")])
+
+
 # ---------------------------------------------------------------- solvers
 def rhs_for(H, W, fmu, fmv, seed):
     """A consistent right-hand side: the divergence of a random velocity."""
