@@ -2158,13 +2158,40 @@ __global__ __launch_bounds__(NB_T) void k_rhs_norm(const T* __restrict__ rhs,
 }
 
 // Sum n partials in a fixed order into out[slot].
+// Convergence state of a tolerance-driven solve, kept on the device so that
+// the host can queue the next iteration before it knows the outcome of the
+// previous one: every launch of an iteration is gated on `active`.
+struct SolveState {
+  int active;   // 1 while the solve continues
+  int cycles;   // refinements (mixed) or V-cycles (native) queued to run
+  double rel;   // relative residual of the last check
+};
+
+// With `init`, also starts a gated solve's state (active, no cycle yet).
 __global__ __launch_bounds__(1024) void k_sum_partials(const double* __restrict__ partial, int n,
-                                                       double* __restrict__ out, int slot) {
+                                                       double* __restrict__ out, int slot,
+                                                       SolveState* __restrict__ init) {
   __shared__ double red[1024 / 64];
+  // the partials loaded in batches before any is added, added in the order
+  // of one after another
+  constexpr int B = 16;
   double acc = 0.0;
-  for (int k = threadIdx.x; k < n; k += 1024) acc += partial[k];
+  for (int k0 = threadIdx.x; k0 < n; k0 += B * 1024) {
+    double v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int k = k0 + b * 1024;
+      v[b] = partial[k < n ? k : n - 1];
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+      if (k0 + b * 1024 < n) acc += v[b];
+  }
   const double tot = block_sum<1024>(acc, red);
-  if (threadIdx.x == 0) out[slot] = tot;
+  if (threadIdx.x == 0) {
+    out[slot] = tot;
+    if (init != nullptr) *init = SolveState{1, 0, -1.0};
+  }
 }
 
 template <typename T>
@@ -2174,15 +2201,6 @@ __global__ void k_add_correction(T* __restrict__ p, const float* __restrict__ e,
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = p[k] + (T)e[k];
 }
-
-// Convergence state of a tolerance-driven solve, kept on the device so that
-// the host can queue the next iteration before it knows the outcome of the
-// previous one: every launch of an iteration is gated on `active`.
-struct SolveState {
-  int active;   // 1 while the solve continues
-  int cycles;   // refinements (mixed) or V-cycles (native) queued to run
-  double rel;   // relative residual of the last check
-};
 
 // A slot of the pinned host ring: the state after one check, then the
 // check's sequence number, written last (after a system-scope fence), so
@@ -2773,7 +2791,7 @@ int residual_norm2_dev(const T* p, const T* rhs, const uint8_t* flags, int H, in
   int rc = check_launch("k_residual_norm");
   if (rc) return rc;
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partial, (int)(g.x * g.y),
-                     result, slot);
+                     result, slot, nullptr);
   return check_launch("k_sum_partials");
 }
 
@@ -2791,17 +2809,20 @@ int residual_partials_dev(const T* p, const T* rhs, const uint8_t* flags, int H,
   return check_launch("k_residual_norm");
 }
 
+// ||alpha rhs||^2 over the fluid cells into result[slot] (with `defect`, also
+// the defect of p = 0; with `init`, the reduction also starts a gated solve's
+// state, in place of a k_state_init launch).
 template <typename T>
 int rhs_norm2_dev(const T* rhs, const uint8_t* flags, int H, int W, double alpha,
                   double* partial, double* result, int slot, hipStream_t s,
-                  float* defect = nullptr) {
+                  float* defect = nullptr, SolveState* init = nullptr) {
   dim3 g(cdiv(W, NB_X), cdiv(H, NB_Y));
   hipLaunchKernelGGL(k_rhs_norm<T>, g, dim3(NB_T), 0, s, rhs, flags, H, W, alpha, partial,
                      defect);
   int rc = check_launch("k_rhs_norm");
   if (rc) return rc;
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, partial, (int)(g.x * g.y),
-                     result, slot);
+                     result, slot, init);
   return check_launch("k_sum_partials");
 }
 
@@ -3491,7 +3512,7 @@ int read_norm(smk_mg* mg, int nparts, const T* p, const T* div, double alpha, do
                                nullptr, s);
   } else {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, mg->partial, nparts,
-                       mg->result, 0);
+                       mg->result, 0, nullptr);
     rc = check_launch("k_sum_partials");
   }
   if (rc) return rc;
@@ -3551,13 +3572,11 @@ int mg_solve_native(smk_mg* mg, T* p, const T* div, double rho, double h, double
   }
   if (gated) {
     // norm of the initial guess, then V-cycle + norm per gated iteration
-    rc = rhs_norm2_dev<T>(div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 1, s);
+    rc = rhs_norm2_dev<T>(div, L0.flags, mg->H, mg->W, alpha, mg->partial, mg->result, 1, s,
+                          nullptr, mg->d_state);
     if (rc) return rc;
     MgRun grun = run;
     grun.gate = &mg->d_state->active;
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, s, mg->d_state);
-    rc = check_launch("k_state_init");
-    if (rc) return rc;
     SolveState fin{};
     rc = run_gated(mg, prm->v_cycles + 1, prm->tolerance,
                    [&](int k, RingSlot* slot, unsigned sq) -> int {
@@ -3757,7 +3776,7 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
   const bool gated = gateable(mg, run, 1);
   float* const defect0 = reinterpret_cast<float*>(L0.rhs32);
   rc = rhs_norm2_dev<double>(div, L0.flags, H, W, alpha, mg->partial, mg->result, 1, s,
-                             zg && gated ? defect0 : nullptr);
+                             zg && gated ? defect0 : nullptr, gated ? mg->d_state : nullptr);
   if (rc) return rc;
   if (zg && !gated) {
     hipLaunchKernelGGL(k_fill_gated<double>, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, 0.0,
@@ -3773,9 +3792,6 @@ int mg_solve_mixed(smk_mg* mg, double* p, const double* div, double rho, double 
     // FP32 V-cycles, p += e, defect + check -- all gated on the device state
     MgRun grun = run;
     grun.gate = &mg->d_state->active;
-    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, s, mg->d_state);
-    rc = check_launch("k_state_init");
-    if (rc) return rc;
     const int nparts = norm_blocks(H, W);
     // refinement k reads p from buf[(k-1) & 1] and writes buf[k & 1] (the
     // correction and the defect in one pass); refinements gated off write
