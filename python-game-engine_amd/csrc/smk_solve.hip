@@ -1815,22 +1815,26 @@ __global__ __launch_bounds__(EX_NT) void k_apply_exact(const float* __restrict__
   const int i0 = row0 + r4;
   float acc[4] = {0.f, 0.f, 0.f, 0.f};
   if ((n & 3) == 0) {  // rows i0 .. i0 + 3 all inside or all outside; 16-byte aligned
+    // every load at an index clamped into the arrays, all issued before any
+    // is used; columns past n add nothing, rows past n are not stored
+    const int ic = i0 < n ? i0 : 0;
     float4 m[EX_JMAX];
-    float bj[EX_JMAX];
+    float bv[EX_JMAX];
 #pragma unroll
     for (int k = 0; k < EX_JMAX; ++k) {
       const int j = set + k * EX_SETS;
-      const bool ok = j < n && i0 < n;
-      m[k] = ok ? *reinterpret_cast<const float4*>(op + (size_t)j * n + i0)
-                : make_float4(0.f, 0.f, 0.f, 0.f);
-      bj[k] = j < n ? alpha * b[j] : 0.f;
+      const int jc = j < n ? j : n - 1;
+      m[k] = *reinterpret_cast<const float4*>(op + (size_t)jc * n + ic);
+      bv[k] = b[jc];
     }
 #pragma unroll
     for (int k = 0; k < EX_JMAX; ++k) {
-      acc[0] += m[k].x * bj[k];
-      acc[1] += m[k].y * bj[k];
-      acc[2] += m[k].z * bj[k];
-      acc[3] += m[k].w * bj[k];
+      if (set + k * EX_SETS >= n) continue;
+      const float bj = alpha * bv[k];
+      acc[0] += m[k].x * bj;
+      acc[1] += m[k].y * bj;
+      acc[2] += m[k].z * bj;
+      acc[3] += m[k].w * bj;
     }
   } else {  // row by row
 #pragma unroll
