@@ -606,7 +606,7 @@ def main():
                     help="keep the default tile variants (profiled runs: a profiler's overhead "
                          "distorts the autotuner's timings)")
     ap.add_argument("--variants", default="fp64,mixed")
-    ap.add_argument("--mixed-inner", type=int, default=2,
+    ap.add_argument("--mixed-inner", type=int, default=1,
                     help="FP32 V-cycles per FP64 refinement in the mixed variant")
     ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator", "exact", "exact_1024"],
                     default="exact_1024",
