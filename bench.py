@@ -216,7 +216,7 @@ def traffic_from_pmc(fetch_csv: str, write_csv: str, copy_bytes: int = 1 << 30,
 
 def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3,
                 inner: int = 1, coarsest: str = "sweeps", arith: str = "float64",
-                tune: bool = True, graphs: bool = True):
+                tune: bool = True, graphs: bool = True, omega: float = None):
     """Warm-up steps, the autotuner's choice of tile variants (same results,
     timed on this simulation from a saved state), ``steps`` timed steps,
     then as many steps with the stages around the solve replayed from HIP
@@ -225,10 +225,11 @@ def run_variant(scene, precision: str, steps: int, warmup: int, profile: int = 3
 
     from cfd_amd import scenes, tuning
 
-    sim = scenes.make_simulation(scene, dtype="float64", device="cuda",
-                                 solver_overrides={"precision": precision, "inner_cycles": inner,
-                                                   "coarsest_solve": coarsest,
-                                                   "inner_arithmetic": arith})
+    over = {"precision": precision, "inner_cycles": inner, "coarsest_solve": coarsest,
+            "inner_arithmetic": arith}
+    if omega is not None:  # the smoother's SOR weight (else the scene's)
+        over["smoother_omega"] = omega
+    sim = scenes.make_simulation(scene, dtype="float64", device="cuda", solver_overrides=over)
     torch.cuda.synchronize()
     for _ in range(warmup):
         sim.substep(scene.dt)
@@ -606,7 +607,10 @@ def main():
                     help="keep the default tile variants (profiled runs: a profiler's overhead "
                          "distorts the autotuner's timings)")
     ap.add_argument("--variants", default="fp64,mixed")
-    ap.add_argument("--mixed-inner", type=int, default=1,
+    ap.add_argument("--mixed-omega", type=float, default=1.2,
+                    help="SOR weight of the mixed variant's FP32 V-cycle smoother (the FP64 "
+                         "variant keeps the workload's red-black Gauss-Seidel, 1.0)")
+    ap.add_argument("--mixed-inner", type=int, default=2,
                     help="FP32 V-cycles per FP64 refinement in the mixed variant")
     ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator", "exact", "exact_1024"],
                     default="exact_1024",
@@ -699,7 +703,8 @@ def main():
                     profile=0, tune=False, graphs=False,
                     inner=args.mixed_inner if var == "mixed" else 1,
                     coarsest=args.mixed_coarsest if var == "mixed" else "sweeps",
-                    arith=args.mixed_arith if var == "mixed" else "float64")
+                    arith=args.mixed_arith if var == "mixed" else "float64",
+                    omega=args.mixed_omega if var == "mixed" else None)
         calibrate_copy()
         print(json.dumps({"pmc_run": var, "steps": args.warmup + args.steps}), flush=True)
         return
@@ -709,10 +714,12 @@ def main():
         inner = args.mixed_inner if var == "mixed" else 1
         coarsest = args.mixed_coarsest if var == "mixed" else "sweeps"
         arith = args.mixed_arith if var == "mixed" else "float64"
+        omega = args.mixed_omega if var == "mixed" else None
         r, sims[var] = run_variant(scene, "mixed" if var == "mixed" else "native", args.steps,
                                    args.warmup, inner=inner, coarsest=coarsest, arith=arith,
-                                   tune=not args.no_tune)
+                                   tune=not args.no_tune, omega=omega)
         r["inner_cycles_per_refinement"] = inner
+        r["inner_smoother_omega"] = omega
         r["coarsest_solve"] = coarsest
         r["inner_arithmetic"] = arith if var == "mixed" else None
         ex_level, ex_cells = (exact_level_of(args.N) if var == "mixed" and
@@ -779,7 +786,8 @@ def main():
                    "solver": "multigrid V(2,2) RB-GS, 8 levels, coarsest 50 sweeps, tol 1e-8, "
                              "rediscretized coarse scale" + (
                                  {"exact": "; mixed: FP32 V-cycles, 16x16 level solved exactly",
-                                  "exact_1024": "; mixed: FP32 V-cycles to the 32x32 level, "
+                                  "exact_1024": "; mixed: FP32 V(2,2) cycles, red-black SOR "
+                                                f"omega {args.mixed_omega:g}, to the 32x32 level, "
                                                 "solved exactly"}.get(
                                      results[main_var].get("coarsest_solve"), "")
                                  if main_var == "mixed" else ""),

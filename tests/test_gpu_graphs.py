@@ -32,8 +32,9 @@ def _same(a, b, where):
 
 
 @pytest.mark.parametrize("overrides", [None] + [
-    {"precision": "mixed", "inner_cycles": ic, "coarsest_solve": cs, "inner_arithmetic": "float32"}
-    for cs, ic in (("exact", 2), ("exact_1024", 1))])
+    {"precision": "mixed", "inner_cycles": ic, "coarsest_solve": cs, "inner_arithmetic": "float32",
+     "smoother_omega": w}
+    for cs, ic, w in (("exact", 2, 1.0), ("exact_1024", 2, 1.2))])
 def test_graph_stages_match_eager_steps(overrides):
     sc = scenes.plume_2048_mg(N=256, steps=8)
     a, b = _pair(sc, overrides)

@@ -9,7 +9,8 @@ from cfd_amd.backends.hip import _lib
 
 pytestmark = pytest.mark.gpu
 
-MIXED = {"precision": "mixed", "inner_cycles": 1, "coarsest_solve": "exact_1024",
+MIXED = {"precision": "mixed", "inner_cycles": 2, "smoother_omega": 1.2,
+         "coarsest_solve": "exact_1024",
          "inner_arithmetic": "float32"}
 FIELDS = ("density", "velocity_u", "velocity_v", "pressure", "divergence", "pressure_residual")
 
