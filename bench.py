@@ -427,7 +427,7 @@ def bench_small_configs(steps_c1: int = 100, steps_c2: int = 20) -> dict:
     return out
 
 
-def bench_batched(count: int, steps: int) -> dict:
+def bench_batched(count: int, steps: int, omega: float = 1.0) -> dict:
     """The uneven-scenes workload: ``count`` scenes (N log-uniform in
     [64, 1024], multiple of 16, aspect 1:1 or 2:1, seeds 4000+s), one
     batch, FP32 V-cycles in FP64 defect correction to 1e-6 per scene; the
@@ -438,7 +438,7 @@ def bench_batched(count: int, steps: int) -> dict:
     from cfd_amd.batch import BatchedSmoke
 
     sl = scenes.uneven_scenes(count=count, steps=steps)
-    b = BatchedSmoke(sl, tolerance=1e-6)
+    b = BatchedSmoke(sl, tolerance=1e-6, smoother_omega=omega)
     b.step()
     tune = b.autotune(tile_rows=(48, 32), smoother_iterations=(1, 2, 3),
                       persistent_blocks=(2048, 4096), repeats=1)
@@ -610,6 +610,8 @@ def main():
     ap.add_argument("--mixed-omega", type=float, default=1.2,
                     help="SOR weight of the mixed variant's FP32 V-cycle smoother (the FP64 "
                          "variant keeps the workload's red-black Gauss-Seidel, 1.0)")
+    ap.add_argument("--batch-omega", type=float, default=1.2,
+                    help="SOR weight of the uneven-batch workload's FP32 V-cycle smoother")
     ap.add_argument("--mixed-inner", type=int, default=2,
                     help="FP32 V-cycles per FP64 refinement in the mixed variant")
     ap.add_argument("--mixed-coarsest", choices=["sweeps", "operator", "exact", "exact_1024"],
@@ -686,7 +688,8 @@ def main():
         sys.exit(1)
 
     if args.only == "batched":
-        print(json.dumps(bench_batched(args.batched_scenes or 96, args.steps)), flush=True)
+        print(json.dumps(bench_batched(args.batched_scenes or 96, args.steps, args.batch_omega)),
+              flush=True)
         return
     if args.only == "small":
         print(json.dumps(bench_small_configs()), flush=True)
@@ -762,7 +765,7 @@ def main():
         log(f"small configs: {json.dumps({k: extra[k] for k in ('plume_128_jacobi', 'plume_512_rbsor')})}")
     if args.batched_scenes > 0:
         log(f"batched scenes: {args.batched_scenes}")
-        extra["batched_scenes"] = bench_batched(args.batched_scenes, 20)
+        extra["batched_scenes"] = bench_batched(args.batched_scenes, 20, args.batch_omega)
         log(f"batched scenes: {json.dumps(extra['batched_scenes'])}")
     if args.diff_steps > 0:
         log(f"differentiable rollout: {args.diff_steps} steps")

@@ -429,6 +429,9 @@ typedef struct smk_batch_params {
   int use_graphs;              /* replay stages from HIP graphs           */
   int tile_rows;               /* smoother tile height: 48 or 32          */
   int persistent_blocks;       /* workgroups per persistent launch (0: auto) */
+  double smoother_omega;       /* SOR weight of the smoother on every level
+                                  but the coarsest; <= 0: 1.0 (Gauss-Seidel)
+                                  (new in ABI 6)                           */
 } smk_batch_params;
 
 typedef struct smk_batch_info {

@@ -14,7 +14,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "..", "lib", "libsmoke_hip.so"))
 
-ABI_VERSION = 5  # smk_abi_version() of the library these bindings describe
+ABI_VERSION = 6  # smk_abi_version() of the library these bindings describe
 
 SMK_F64 = 0
 SMK_F32 = 1
@@ -103,7 +103,7 @@ class BatchParams(ctypes.Structure):
         ("coarsest_iterations", ctypes.c_int), ("coarse_scale", ctypes.c_int),
         ("coarsest_omega", ctypes.c_double), ("tolerance", ctypes.c_double), ("max_cycles", ctypes.c_int),
         ("use_graphs", ctypes.c_int), ("tile_rows", ctypes.c_int),
-        ("persistent_blocks", ctypes.c_int),
+        ("persistent_blocks", ctypes.c_int), ("smoother_omega", ctypes.c_double),
     ]
 
 

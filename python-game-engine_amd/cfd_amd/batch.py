@@ -39,7 +39,7 @@ class BatchedSmoke:
     def __init__(self, scenes, *, tolerance: float = 1e-6, max_cycles: int = 100,
                  smoother_iterations: int = 2, coarsest_iterations: int = 50,
                  coarsest_omega: float = -1.0, use_graphs: bool = True, tile_rows: int = 48,
-                 persistent_blocks: int = 0,
+                 persistent_blocks: int = 0, smoother_omega: float = 1.0,
                  fluid_density: float = 1.0, device="cuda"):
         if not scenes:
             raise ValueError("need at least one scene")
@@ -99,7 +99,8 @@ class BatchedSmoke:
             coarse_scale=_lib.SMK_COARSE_REDISCRETIZED, coarsest_omega=float(coarsest_omega),
             tolerance=float(tolerance),
             max_cycles=int(max_cycles), use_graphs=1 if use_graphs else 0,
-            tile_rows=int(tile_rows), persistent_blocks=int(persistent_blocks))
+            tile_rows=int(tile_rows), persistent_blocks=int(persistent_blocks),
+            smoother_omega=float(smoother_omega))
         self._arr = arr
         h = ctypes.c_void_p()
         torch.cuda.synchronize(self.device)

@@ -257,7 +257,7 @@ static int batch_build_vcycle(smk_batch* b, hipStream_t s) {
       a.H = L.H;
       a.W = L.W;
       a.alpha = alpha;
-      a.omega = 1.0;
+      a.omega = b->prm.smoother_omega > 0.0 ? b->prm.smoother_omega : 1.0;
       a.npass = npass;
       a.first_colour = 0;
       a.R = npass + 1;
@@ -274,7 +274,7 @@ static int batch_build_vcycle(smk_batch* b, hipStream_t s) {
       q.H = L.H;
       q.W = L.W;
       q.alpha = alpha;
-      q.omega = 1.0;
+      q.omega = a.omega;
       q.npass = npass;
       q.first_colour = 0;
       q.R = npass;
@@ -305,7 +305,7 @@ static int batch_build_vcycle(smk_batch* b, hipStream_t s) {
       m.W[k] = L.W;
       m.flags[k] = L.flags;
       m.alpha[k] = calc_alpha(hl, rho, dt);
-      m.omega[k] = 1.0;
+      m.omega[k] = b->prm.smoother_omega > 0.0 ? b->prm.smoother_omega : 1.0;
       if (k == m.nl - 1 && b->prm.coarsest_omega != 0.0)
         m.omega[k] = b->prm.coarsest_omega > 0.0 ? b->prm.coarsest_omega
                                                  : optimal_omega(L.H > L.W ? L.H : L.W);

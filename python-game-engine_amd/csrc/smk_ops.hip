@@ -816,7 +816,7 @@ using namespace smk;
 extern "C" {
 
 const char* smk_last_error(void) { return g_last_error.c_str(); }
-int smk_abi_version(void) { return 5; }
+int smk_abi_version(void) { return 6; }
 
 int smk_face_masks_from_solid(const uint8_t* solid, int H, int W, uint8_t* fmu,
                               uint8_t* fmv, void* stream) {

@@ -14,10 +14,12 @@ def _scenes(count=5, n_max=192, seed=4100):
     return scenes.uneven_scenes(count=count, steps=3, base_seed=seed, n_min=48, n_max=n_max)
 
 
-def _single(sc, steps):
+def _single(sc, steps, omega=None):
     from cfd_amd import scenes
-    sim = scenes.make_simulation(sc, dtype="float64", device="cuda",
-                                 solver_overrides={"precision": "mixed", "inner_cycles": 1})
+    over = {"precision": "mixed", "inner_cycles": 1}
+    if omega is not None:
+        over["smoother_omega"] = omega
+    sim = scenes.make_simulation(sc, dtype="float64", device="cuda", solver_overrides=over)
     for _ in range(steps):
         sim.substep(sc.dt)
     torch.cuda.synchronize()
@@ -29,17 +31,20 @@ def scene_list():
     return _scenes()
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batch_equals_single_scene_steps(scene_list, graphs):
+# omega 1.2: the smoother's SOR weight reaches every batched level as it
+# does the single-scene solve's (smoother_omega)
+@pytest.mark.parametrize("graphs,omega", [(True, None), (False, None), (True, 1.2)])
+def test_batch_equals_single_scene_steps(scene_list, graphs, omega):
     from cfd_amd.batch import BatchedSmoke
     steps = 3
-    b = BatchedSmoke(scene_list, tolerance=1e-6, max_cycles=400, use_graphs=graphs)
+    kw = {} if omega is None else {"smoother_omega": omega}
+    b = BatchedSmoke(scene_list, tolerance=1e-6, max_cycles=400, use_graphs=graphs, **kw)
     for _ in range(steps):
         info = b.step()
         assert info["max_rel_residual"] <= 1e-6
     torch.cuda.synchronize()
     for k, sc in enumerate(scene_list):
-        sim = _single(sc, steps)
+        sim = _single(sc, steps, omega)
         assert torch.equal(b.density(k), sim.density), f"scene {k} ({sc.H}x{sc.W}) density"
         u, v = b.velocity(k)
         assert torch.equal(u, sim.velocity_u), f"scene {k} u"

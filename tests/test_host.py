@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, s), f"{s} declared in smoke_hip.h but not exported"
     # the ctypes table types exactly the declared entry points
     assert sorted(_lib.SIGNATURES) == syms
-    assert L.smk_abi_version() == _lib.ABI_VERSION == 5
+    assert L.smk_abi_version() == _lib.ABI_VERSION == 6
 
 
 def test_library_rejects_bad_arguments_without_a_gpu():
